@@ -1,0 +1,186 @@
+/*
+ * C ABI of the MI355X OP2 runtime, libop2hip.so.
+ *
+ * Every entry point is extern "C" with plain pointers, sizes and the OP2
+ * handle types of op_lib_core.h.  Each one names the reference interface it
+ * replaces (file:line in the reference tree).  The C++ API of op_lib.h and
+ * op_seq.h is a header-only layer over these symbols; a foreign-function
+ * binding (ctypes, cgo, JNI) binds these directly.
+ *
+ * Errors: by default an error prints a message and calls exit(), as the
+ * reference does (common/op_lib_core.c:260-264, common/op_lib.cu:60-80).
+ * op2_set_error_mode(OP2_ERRORS_RETURN) makes entry points record the
+ * message (op2_last_error) and return an error value instead.
+ */
+#ifndef OP2_HIP_H
+#define OP2_HIP_H
+
+#include "op_lib_core.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------------------
+ * Reference C-linkage entry points (common/op_lib.h:101-153)
+ */
+
+/* common/op_lib_core.c:68-92: diagnostics level; argv OP_BLOCK_SIZE=,
+ * OP_PART_SIZE=, OP_CACHE_LINE_SIZE= */
+void op_init_core(int argc, char **argv, int diags);
+
+/* common/op_lib_core.c:94-119 (size must be > 0) */
+op_set op_decl_set(int size, char const *name);
+
+/* common/op_lib_core.c:121-169 (entries checked in range); the map is also
+ * copied to the device */
+op_map op_decl_map(op_set from, op_set to, int dim, int *imap, char const *name);
+
+/* common/op_lib_core.c:171-207: registers the dat without a device copy */
+op_dat op_decl_dat_core(op_set set, int dim, char const *type, int size,
+                        char *data, char const *name);
+
+/* common/op_lib_core.c:266-305 */
+void op_arg_check(op_set set, int m, op_arg arg, int *ninds, char const *name);
+
+/* common/op_lib_core.c:307-326 */
+op_arg op_arg_dat(op_dat dat, int idx, op_map map, int dim, char const *type,
+                  op_access acc);
+
+/* common/op_lib_core.c:329-345 */
+op_arg op_arg_gbl_core(char *data, int dim, char const *type, op_access acc);
+
+/* common/op_lib_core.c:351-379 */
+void op_diagnostic_output(void);
+
+/* common/op_lib_core.c:381-403: count, seconds, GB/s per loop; seconds are
+ * device time of the loop's kernels (HIP events) */
+void op_timing_output(void);
+
+/* common/op_lib_core.c:627-1089: returns the runtime's plan for the loop as
+ * an op_plan view (host arrays of the block layout and colouring) */
+op_plan *op_plan_core(char const *name, op_set set, int part_size, int nargs,
+                      op_arg *args, int ninds, int *inds);
+
+/* common/op_lib_core.c:440-445 */
+void op_timers(double *cpu, double *et);
+
+/* common/op_lib_core.c:405-427 */
+void op_timing_realloc(int kernel);
+
+/* common/op_lib_core.c:209-254 */
+void op_exit_core(void);
+
+/* reference globals (common/op_lib.h:259-271) */
+extern int OP_part_size, OP_block_size, OP_cache_line_size;
+extern int OP_set_index, OP_set_max, OP_map_index, OP_map_max, OP_dat_index,
+    OP_dat_max, OP_plan_index, OP_plan_max, OP_kern_max;
+extern op_set *OP_set_list;
+extern op_map *OP_map_list;
+extern op_dat *OP_dat_list;
+extern op_kernel *OP_kernels;
+
+/* ---------------------------------------------------------------------------
+ * C-ABI forms of the reference's C++-linkage entry points
+ */
+
+/* op_init, common/op_lib.cu:128-143: op_init_core + HIP device 0, runtime
+ * stream and events */
+void op2_init(int argc, char **argv, int diags);
+
+/* op_decl_dat_char, common/op_lib.cu:145-153: registers the dat and copies
+ * it to device memory; the host array stays the user's */
+op_dat op2_decl_dat_char(op_set set, int dim, char const *type, int size,
+                         char *data, char const *name);
+
+/* op_decl_const_char, airfoil/airfoil_kernels.cu:50-53: stores a named
+ * constant (dim elements of size bytes) for the device loops */
+void op2_decl_const_char(int dim, char const *type, int size, char *data,
+                         char const *name);
+
+/* op_fetch_data, common/op_lib.cu:116-121: device -> user's host array */
+void op2_fetch_data(op_dat dat);
+
+/* op_plan_get, common/op_lib.cu:155-184 */
+op_plan *op2_plan_get(char const *name, op_set set, int part_size, int nargs,
+                      op_arg *args, int ninds, int *inds);
+
+/* op_exit, common/op_lib.cu:186-209: frees plans, device data, registries */
+void op2_exit(void);
+
+/* op_par_loop (common/op_seq.h:26-1340) and the generated per-loop stubs
+ * op_par_loop_<name> (airfoil/airfoil_op.cpp:64-99, jac1/jac_op.cpp): runs
+ * the loop `name` over `set` on the device.  The loop is identified by its
+ * name and argument signature; loops this runtime has no device kernel for
+ * are an error.  OP_GBL arguments are read on entry and, for OP_INC, OP_MIN
+ * and OP_MAX, written back to the caller's variable before return (unless
+ * reductions are deferred, see op2_set_deferred_reductions).
+ * Returns 0 on success. */
+int op2_par_loop(char const *name, op_set set, int nargs, op_arg *args);
+
+/* ---------------------------------------------------------------------------
+ * Runtime controls (no counterpart in the reference)
+ */
+
+#define OP2_ERRORS_EXIT 0
+#define OP2_ERRORS_RETURN 1
+int op2_set_error_mode(int mode);
+/* message of the last error ("" if none since op2_clear_error) */
+char const *op2_last_error(void);
+void op2_clear_error(void);
+
+/* Execution strategies for indirect loops with OP_INC arguments. */
+#define OP2_STRATEGY_AUTO 0
+/* block colours (one launch per colour) and thread colours inside a block:
+ * conflict-free, deterministic */
+#define OP2_STRATEGY_COLOURED 1
+/* one launch; LDS atomics inside a block, global atomics across blocks */
+#define OP2_STRATEGY_ATOMIC 2
+
+/* Per-loop tuning: strategy, part size (elements per block, 0 = default)
+ * and threads per block (0 = default).  name NULL applies to every loop. */
+int op2_set_loop_config(char const *name, int strategy, int part_size,
+                        int block_size);
+
+/* Device seconds, call count and algorithmic bytes per call of a loop
+ * (device time from HIP events on the runtime stream).  Returns 0 if the
+ * loop has run. */
+int op2_loop_stats(char const *name, int *count, double *seconds,
+                   double *bytes_per_call);
+void op2_timing_reset(void);
+
+/* The runtime's HIP stream (hipStream_t). */
+void *op2_stream(void);
+/* Waits for all work on the runtime stream. */
+int op2_sync(void);
+
+/* Device pointer of a dat (for zero-copy interop). */
+void *op2_dat_device_ptr(op_dat dat);
+
+/* Plan introspection for the plan of the given strategy (COLOURED or
+ * ATOMIC) and part size (0 = 512): block count, block colours, largest block, largest
+ * number of staged elements of one block (summed over its maps), mean
+ * thread colours, and the number of violations found by the exact plan
+ * verifier (0 = every block and thread colouring is valid and every local
+ * index maps back to the user's map). */
+typedef struct {
+  int nblocks, ncolors, max_block, max_staged, strategy, part_size;
+  double mean_thread_colours;
+  long long verify_errors;
+} op2_plan_summary;
+int op2_plan_summary_get(char const *name, op_set set, int nargs, op_arg *args, int strategy,
+                         int part_size, op2_plan_summary *out);
+
+/* Plan arrays for independent checking: perm[set size] (element at each
+ * position), thrcol[set size] (thread colour of each position), blk_off
+ * [nblocks+1] (positions of each block) and blkcol[nblocks]; any pointer may
+ * be NULL.  Returns nblocks, or -1 on error.  Plans are built and verified
+ * on the host, so this needs no GPU. */
+int op2_plan_export(char const *name, op_set set, int nargs, op_arg *args, int strategy,
+                    int part_size, int *perm, int *thrcol, int *blk_off, int *blkcol);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,189 @@
+"""The airfoil and Jacobi programs over the OP2 API, run on the GPU.
+
+`run_airfoil` is the time-march of the reference airfoil program
+(airfoil/airfoil.cpp:152-257) written against op2.py; `run_jacobi` is the
+iteration of jac1/jac.cpp:122-169.  Both declare everything through the
+public API, so they exercise the same path a C/C++ program takes.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+from . import op2
+from .op2 import (OP_ID, OP_INC, OP_MAX, OP_READ, OP_RW, OP_WRITE, op_arg_dat, op_arg_gbl,
+                  op_decl_const2, op_decl_dat, op_decl_map, op_decl_set, op_par_loop)
+
+
+class Airfoil:
+    """Sets, maps and dats of one airfoil mesh, declared as
+    airfoil/airfoil.cpp:165-189 declares them."""
+
+    def __init__(self, mesh, consts: np.ndarray, dtype: str = "double", q0=None, res0=None):
+        self.real = np.float64 if dtype == "double" else np.float32
+        self.t = dtype
+        self.mesh = mesh
+        self.ncell = mesh.ncell
+        real = self.real
+        self.x = np.ascontiguousarray(mesh.x.astype(real).reshape(-1))
+        qinf = np.asarray(consts[6:10], dtype=real)
+        self.q = np.ascontiguousarray(np.tile(qinf, mesh.ncell))
+        if q0 is not None:
+            self.q[:] = np.asarray(q0, dtype=real).reshape(-1)
+        self.qold = np.zeros_like(self.q)
+        self.res = np.zeros_like(self.q)
+        if res0 is not None:
+            self.res[:] = np.asarray(res0, dtype=real).reshape(-1)
+        self.adt = np.zeros(mesh.ncell, dtype=real)
+
+        self.nodes = op_decl_set(mesh.nnode, "nodes")
+        self.edges = op_decl_set(mesh.nedge, "edges")
+        self.bedges = op_decl_set(mesh.nbedge, "bedges")
+        self.cells = op_decl_set(mesh.ncell, "cells")
+        self.pedge = op_decl_map(self.edges, self.nodes, 2, mesh.edge, "pedge")
+        self.pecell = op_decl_map(self.edges, self.cells, 2, mesh.ecell, "pecell")
+        self.pbedge = op_decl_map(self.bedges, self.nodes, 2, mesh.bedge, "pbedge")
+        self.pbecell = op_decl_map(self.bedges, self.cells, 1, mesh.becell, "pbecell")
+        self.pcell = op_decl_map(self.cells, self.nodes, 4, mesh.cell, "pcell")
+        self.p_bound = op_decl_dat(self.bedges, 1, "int", np.ascontiguousarray(mesh.bound), "p_bound")
+        self.p_x = op_decl_dat(self.nodes, 2, dtype, self.x, "p_x")
+        self.p_q = op_decl_dat(self.cells, 4, dtype, self.q, "p_q")
+        self.p_qold = op_decl_dat(self.cells, 4, dtype, self.qold, "p_qold")
+        self.p_adt = op_decl_dat(self.cells, 1, dtype, self.adt, "p_adt")
+        self.p_res = op_decl_dat(self.cells, 4, dtype, self.res, "p_res")
+        names = ("gam", "gm1", "cfl", "eps", "mach", "alpha")
+        for i, n in enumerate(names):
+            op_decl_const2(n, 1, dtype, np.array([consts[i]], dtype=real))
+        op_decl_const2("qinf", 4, dtype, np.asarray(consts[6:10], dtype=real))
+        self.rms = np.zeros(1, dtype=np.float64)
+
+    # the five loops, with the arguments of airfoil/airfoil.cpp:201-248
+    def save_soln(self):
+        t = self.t
+        op_par_loop(op2.save_soln, "save_soln", self.cells,
+                    op_arg_dat(self.p_q, -1, OP_ID, 4, t, OP_READ),
+                    op_arg_dat(self.p_qold, -1, OP_ID, 4, t, OP_WRITE))
+
+    def adt_calc(self):
+        t = self.t
+        op_par_loop(op2.adt_calc, "adt_calc", self.cells,
+                    op_arg_dat(self.p_x, 0, self.pcell, 2, t, OP_READ),
+                    op_arg_dat(self.p_x, 1, self.pcell, 2, t, OP_READ),
+                    op_arg_dat(self.p_x, 2, self.pcell, 2, t, OP_READ),
+                    op_arg_dat(self.p_x, 3, self.pcell, 2, t, OP_READ),
+                    op_arg_dat(self.p_q, -1, OP_ID, 4, t, OP_READ),
+                    op_arg_dat(self.p_adt, -1, OP_ID, 1, t, OP_WRITE))
+
+    def res_calc_args(self):
+        t = self.t
+        return (op_arg_dat(self.p_x, 0, self.pedge, 2, t, OP_READ),
+                op_arg_dat(self.p_x, 1, self.pedge, 2, t, OP_READ),
+                op_arg_dat(self.p_q, 0, self.pecell, 4, t, OP_READ),
+                op_arg_dat(self.p_q, 1, self.pecell, 4, t, OP_READ),
+                op_arg_dat(self.p_adt, 0, self.pecell, 1, t, OP_READ),
+                op_arg_dat(self.p_adt, 1, self.pecell, 1, t, OP_READ),
+                op_arg_dat(self.p_res, 0, self.pecell, 4, t, OP_INC),
+                op_arg_dat(self.p_res, 1, self.pecell, 4, t, OP_INC))
+
+    def res_calc(self):
+        op_par_loop(op2.res_calc, "res_calc", self.edges, *self.res_calc_args())
+
+    def bres_calc(self):
+        t = self.t
+        op_par_loop(op2.bres_calc, "bres_calc", self.bedges,
+                    op_arg_dat(self.p_x, 0, self.pbedge, 2, t, OP_READ),
+                    op_arg_dat(self.p_x, 1, self.pbedge, 2, t, OP_READ),
+                    op_arg_dat(self.p_q, 0, self.pbecell, 4, t, OP_READ),
+                    op_arg_dat(self.p_adt, 0, self.pbecell, 1, t, OP_READ),
+                    op_arg_dat(self.p_res, 0, self.pbecell, 4, t, OP_INC),
+                    op_arg_dat(self.p_bound, -1, OP_ID, 1, "int", OP_READ))
+
+    def update(self) -> float:
+        t = self.t
+        self.rms[0] = 0.0
+        op_par_loop(op2.update, "update", self.cells,
+                    op_arg_dat(self.p_qold, -1, OP_ID, 4, t, OP_READ),
+                    op_arg_dat(self.p_q, -1, OP_ID, 4, t, OP_WRITE),
+                    op_arg_dat(self.p_res, -1, OP_ID, 4, t, OP_RW),
+                    op_arg_dat(self.p_adt, -1, OP_ID, 1, t, OP_READ),
+                    op_arg_gbl(self.rms, 1, "double", OP_INC))
+        return float(self.rms[0])
+
+    def iterate(self) -> float:
+        """One iteration of airfoil/airfoil.cpp:197-257; returns
+        sqrt(rms / ncell) of the second stage."""
+        self.save_soln()
+        rms = 0.0
+        for _ in range(2):
+            self.adt_calc()
+            self.res_calc()
+            self.bres_calc()
+            rms = self.update()
+        return float(np.sqrt(rms / self.ncell))
+
+    def fetch(self):
+        for d in (self.p_q, self.p_qold, self.p_adt, self.p_res):
+            op2.op_fetch_data(d)
+        return self.q.reshape(-1, 4), self.qold.reshape(-1, 4), self.adt, self.res.reshape(-1, 4)
+
+
+def run_airfoil(mesh, consts, niter: int, dtype: str = "double"):
+    """Declares the mesh, runs `niter` iterations, returns (rms history,
+    q, adt, res) after fetching the dats.  The caller owns op_init/op_exit."""
+    a = Airfoil(mesh, consts, dtype)
+    hist = np.array([a.iterate() for _ in range(niter)])
+    q, _, adt, res = a.fetch()
+    return hist, q.copy(), adt.copy(), res.copy()
+
+
+class Jacobi:
+    """The jac1 program's sets, map and dats (jac1/jac.cpp:128-139)."""
+
+    def __init__(self, prob, alpha: float, dtype: str = "double"):
+        real = np.float64 if dtype == "double" else np.float32
+        self.t = dtype
+        self.real = real
+        self.A = np.ascontiguousarray(prob.A.astype(real))
+        self.r = np.ascontiguousarray(prob.r.astype(real))
+        self.u = np.ascontiguousarray(prob.u.astype(real))
+        self.du = np.ascontiguousarray(prob.du.astype(real))
+        self.nnode = prob.nnode
+        self.nodes = op_decl_set(prob.nnode, "nodes")
+        self.edges = op_decl_set(prob.nedge, "edges")
+        self.ppedge = op_decl_map(self.edges, self.nodes, 2, prob.pp, "ppedge")
+        self.p_A = op_decl_dat(self.edges, 1, dtype, self.A, "p_A")
+        self.p_r = op_decl_dat(self.nodes, 1, dtype, self.r, "p_r")
+        self.p_u = op_decl_dat(self.nodes, 1, dtype, self.u, "p_u")
+        self.p_du = op_decl_dat(self.nodes, 1, dtype, self.du, "p_du")
+        op_decl_const2("alpha", 1, dtype, np.array([alpha], dtype=real))
+
+    def iterate(self, beta: float):
+        """One iteration of jac1/jac.cpp:147-163; returns (u_sum, u_max)."""
+        t = self.t
+        b = np.array([beta], dtype=self.real)
+        op_par_loop(op2.res, "res", self.edges,
+                    op_arg_dat(self.p_A, -1, OP_ID, 1, t, OP_READ),
+                    op_arg_dat(self.p_u, 1, self.ppedge, 1, t, OP_READ),
+                    op_arg_dat(self.p_du, 0, self.ppedge, 1, t, OP_INC),
+                    op_arg_gbl(b, 1, t, OP_READ))
+        u_sum = np.zeros(1, dtype=np.float64)
+        u_max = np.zeros(1, dtype=np.float64)
+        op_par_loop(op2.update, "update", self.nodes,
+                    op_arg_dat(self.p_r, -1, OP_ID, 1, t, OP_READ),
+                    op_arg_dat(self.p_du, -1, OP_ID, 1, t, OP_RW),
+                    op_arg_dat(self.p_u, -1, OP_ID, 1, t, OP_INC),
+                    op_arg_gbl(u_sum, 1, "double", OP_INC),
+                    op_arg_gbl(u_max, 1, "double", OP_MAX))
+        return float(u_sum[0]), float(u_max[0])
+
+
+def run_jacobi(prob, niter: int, alpha: float, beta: float, dtype: str = "double"):
+    j = Jacobi(prob, alpha, dtype)
+    hs, hm = [], []
+    for _ in range(niter):
+        s, m = j.iterate(beta)
+        hs.append(s)
+        hm.append(m)
+    op2.op_fetch_data(j.p_u)
+    op2.op_fetch_data(j.p_du)
+    return np.array(hs), np.array(hm), j.u.copy(), j.du.copy()

@@ -1,0 +1,586 @@
+// OP2 runtime core: registries, declarations, argument checks, constants,
+// timing and the C ABI (see include/op2_hip.h for the entry points and the
+// reference interfaces they replace).
+#include <sys/time.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+
+#include "op2_internal.h"
+
+extern "C" {
+int OP_diags = 0, OP_part_size = 0, OP_block_size = 256, OP_cache_line_size = 128;
+int OP_set_index = 0, OP_set_max = 0, OP_map_index = 0, OP_map_max = 0,
+    OP_dat_index = 0, OP_dat_max = 0, OP_plan_index = 0, OP_plan_max = 0,
+    OP_kern_max = 0;
+op_set *OP_set_list = nullptr;
+op_map *OP_map_list = nullptr;
+op_dat *OP_dat_list = nullptr;
+op_kernel *OP_kernels = nullptr;
+}
+
+namespace op2 {
+
+Runtime &rt() {
+  static Runtime r;
+  return r;
+}
+
+void fail(const char *fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  Runtime &r = rt();
+  r.last_error = buf;
+  if (r.error_mode == OP2_ERRORS_RETURN) throw Error(buf);
+  printf("%s\n", buf);
+  fflush(stdout);
+  exit(-1);
+}
+
+double Const::as_double(int i) const {
+  if (i < 0 || i >= dim) fail("constant index %d out of range (dim %d)", i, dim);
+  const char *p = bytes.data() + (size_t)i * size;
+  if (type == "double") return *(const double *)p;
+  if (type == "float") return *(const float *)p;
+  if (type == "int") return *(const int *)p;
+  fail("constant of type %s cannot be used by the device loops", type.c_str());
+}
+
+void ensure_device() {
+  Runtime &r = rt();
+  if (r.device_ready) return;
+  int n = 0;
+  OP2_HIP_CHECK(hipGetDeviceCount(&n));
+  if (n == 0) fail("op2: no HIP device available");
+  OP2_HIP_CHECK(hipSetDevice(r.device));
+  OP2_HIP_CHECK(hipStreamCreateWithFlags(&r.stream, hipStreamNonBlocking));
+  OP2_HIP_CHECK(hipHostMalloc((void **)&r.h_result, 64 * sizeof(double), hipHostMallocDefault));
+  r.device_ready = true;
+}
+
+void *dev_alloc(size_t bytes) {
+  ensure_device();
+  void *p = nullptr;
+  OP2_HIP_CHECK(hipMalloc(&p, bytes ? bytes : 256));
+  return p;
+}
+
+void dev_free(void *p) {
+  if (p) (void)hipFree(p);
+}
+
+void h2d(void *dst, const void *src, size_t bytes) {
+  if (!bytes) return;
+  OP2_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, rt().stream));
+  OP2_HIP_CHECK(hipStreamSynchronize(rt().stream));
+}
+
+void d2h(void *dst, const void *src, size_t bytes) {
+  if (!bytes) return;
+  OP2_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, rt().stream));
+  OP2_HIP_CHECK(hipStreamSynchronize(rt().stream));
+}
+
+void ensure_map_device(op_map m) {
+  if (m->map_d) return;
+  size_t bytes = sizeof(int) * (size_t)m->dim * m->from->size;
+  m->map_d = (int *)dev_alloc(bytes);
+  h2d(m->map_d, m->map, bytes);
+}
+
+void *reduct_scratch(size_t bytes) {
+  Runtime &r = rt();
+  if (bytes > r.reduct_bytes) {
+    if (r.d_reduct) {
+      OP2_HIP_CHECK(hipStreamSynchronize(r.stream));
+      dev_free(r.d_reduct);
+    }
+    r.reduct_bytes = ((bytes + 255) / 256) * 256 * 2;
+    r.d_reduct = dev_alloc(r.reduct_bytes);
+    OP2_HIP_CHECK(hipMemsetAsync(r.d_reduct, 0, r.reduct_bytes, r.stream));
+  }
+  return r.d_reduct;
+}
+
+const Const &get_const(const char *name) {
+  auto it = rt().consts.find(name);
+  if (it == rt().consts.end())
+    fail("op2: constant \"%s\" has not been declared (op_decl_const)", name);
+  return it->second;
+}
+
+LoopConfig config_for(const std::string &loop) {
+  Runtime &r = rt();
+  LoopConfig c = r.default_config;
+  auto it = r.loop_configs.find(loop);
+  if (it != r.loop_configs.end()) {
+    if (it->second.strategy) c.strategy = it->second.strategy;
+    if (it->second.part_size) c.part_size = it->second.part_size;
+    if (it->second.block_size) c.block_size = it->second.block_size;
+  }
+  return c;
+}
+
+// ---- timing -----------------------------------------------------------------
+
+LoopTiming &timing_for(const std::string &name, double bytes_per_call) {
+  Runtime &r = rt();
+  for (auto &t : r.timings)
+    if (t.name == name) {
+      t.bytes_per_call = bytes_per_call;
+      return t;
+    }
+  r.timings.emplace_back();
+  r.timings.back().name = name;
+  r.timings.back().bytes_per_call = bytes_per_call;
+  return r.timings.back();
+}
+
+static hipEvent_t take_event() {
+  Runtime &r = rt();
+  if (r.event_pool.empty()) {
+    hipEvent_t e;
+    OP2_HIP_CHECK(hipEventCreate(&e));
+    return e;
+  }
+  hipEvent_t e = r.event_pool.back();
+  r.event_pool.pop_back();
+  return e;
+}
+
+void timing_begin(LoopTiming &t, hipEvent_t *start) {
+  (void)t;
+  *start = nullptr;
+  if (!rt().timing_enabled) return;
+  *start = take_event();
+  OP2_HIP_CHECK(hipEventRecord(*start, rt().stream));
+}
+
+void timing_end(LoopTiming &t, hipEvent_t start) {
+  t.count++;
+  if (!start) return;
+  hipEvent_t stop = take_event();
+  OP2_HIP_CHECK(hipEventRecord(stop, rt().stream));
+  t.pending.push_back({start, stop});
+  if (t.pending.size() >= 256) timing_drain();
+}
+
+void timing_drain() {
+  Runtime &r = rt();
+  if (!r.device_ready) return;
+  OP2_HIP_CHECK(hipStreamSynchronize(r.stream));
+  for (auto &t : r.timings) {
+    for (auto &p : t.pending) {
+      float ms = 0;
+      OP2_HIP_CHECK(hipEventElapsedTime(&ms, p.start, p.stop));
+      t.seconds += ms * 1e-3;
+      r.event_pool.push_back(p.start);
+      r.event_pool.push_back(p.stop);
+    }
+    t.pending.clear();
+  }
+}
+
+static void clear_all() {
+  Runtime &r = rt();
+  if (r.device_ready) (void)hipStreamSynchronize(r.stream);
+  for (auto &p : r.plans) p->release();
+  r.plans.clear();
+  for (int i = 0; i < OP_dat_index; i++) {
+    dev_free(OP_dat_list[i]->data_d);
+    free(OP_dat_list[i]);
+  }
+  for (int i = 0; i < OP_map_index; i++) {
+    dev_free(OP_map_list[i]->map_d);
+    free(OP_map_list[i]);
+  }
+  for (int i = 0; i < OP_set_index; i++) free(OP_set_list[i]);
+  free(OP_dat_list);
+  free(OP_map_list);
+  free(OP_set_list);
+  free(OP_kernels);
+  OP_dat_list = nullptr;
+  OP_map_list = nullptr;
+  OP_set_list = nullptr;
+  OP_kernels = nullptr;
+  OP_set_index = OP_set_max = OP_map_index = OP_map_max = 0;
+  OP_dat_index = OP_dat_max = OP_plan_index = OP_plan_max = OP_kern_max = 0;
+  for (auto &t : r.timings)
+    for (auto &p : t.pending) {
+      r.event_pool.push_back(p.start);
+      r.event_pool.push_back(p.stop);
+    }
+  r.timings.clear();
+  r.consts.clear();
+}
+
+}  // namespace op2
+
+using namespace op2;
+
+#define API_BEGIN try {
+#define API_END(ret)          \
+  }                           \
+  catch (const op2::Error &) { \
+    return ret;               \
+  }
+
+template <class T>
+static void grow(T **list, int *max, int index) {
+  if (index == *max) {
+    *max += 10;
+    T *n = (T *)realloc(*list, sizeof(T) * (*max));
+    if (!n) fail(" op2 error -- error reallocating memory");
+    *list = n;
+  }
+}
+
+extern "C" {
+
+void op_init_core(int argc, char **argv, int diags) {
+  API_BEGIN
+  OP_diags = diags;
+#ifdef OP_BLOCK_SIZE
+  OP_block_size = OP_BLOCK_SIZE;
+#endif
+#ifdef OP_PART_SIZE
+  OP_part_size = OP_PART_SIZE;
+#endif
+  for (int n = 1; n < argc; n++) {
+    if (!argv || !argv[n]) continue;
+    if (strncmp(argv[n], "OP_BLOCK_SIZE=", 14) == 0) {
+      OP_block_size = atoi(argv[n] + 14);
+      printf("\n OP_block_size = %d \n", OP_block_size);
+    }
+    if (strncmp(argv[n], "OP_PART_SIZE=", 13) == 0) {
+      OP_part_size = atoi(argv[n] + 13);
+      printf("\n OP_part_size  = %d \n", OP_part_size);
+    }
+    if (strncmp(argv[n], "OP_CACHE_LINE_SIZE=", 19) == 0) {
+      OP_cache_line_size = atoi(argv[n] + 19);
+      printf("\n OP_cache_line_size  = %d \n", OP_cache_line_size);
+    }
+  }
+  rt().initialised = true;
+  API_END()
+}
+
+void op2_init(int argc, char **argv, int diags) {
+  API_BEGIN
+  op_init_core(argc, argv, diags);
+  ensure_device();
+  if (OP_diags > 0) {
+    hipDeviceProp_t prop;
+    OP2_HIP_CHECK(hipGetDeviceProperties(&prop, rt().device));
+    printf("\n Using HIP device: %s (%s)\n", prop.name, prop.gcnArchName);
+  }
+  API_END()
+}
+
+op_set op_decl_set(int size, char const *name) {
+  API_BEGIN
+  if (size <= 0) fail(" op_decl_set error -- negative/zero size for set: %s", name);
+  grow(&OP_set_list, &OP_set_max, OP_set_index);
+  op_set s = (op_set)malloc(sizeof(op_set_core));
+  s->index = OP_set_index;
+  s->size = size;
+  s->name = name;
+  OP_set_list[OP_set_index++] = s;
+  return s;
+  API_END(nullptr)
+}
+
+op_map op_decl_map(op_set from, op_set to, int dim, int *imap, char const *name) {
+  API_BEGIN
+  if (!from) fail(" op_decl_map error -- invalid 'from' set for map %s", name);
+  if (!to) fail("op_decl_map error -- invalid 'to' set for map %s", name);
+  if (dim <= 0) fail("op_decl_map error -- negative/zero dimension for map %s", name);
+  if (!imap) fail("op_decl_map error -- NULL data for map %s", name);
+  for (int d = 0; d < dim; d++)
+    for (int n = 0; n < from->size; n++) {
+      int v = imap[d + (size_t)n * dim];
+      if (v < 0 || v >= to->size)
+        fail("op_decl_map error -- invalid data for map %s\nelement = %d, dimension = %d, map = %d",
+             name, n, d, v);
+    }
+  grow(&OP_map_list, &OP_map_max, OP_map_index);
+  op_map m = (op_map)malloc(sizeof(op_map_core));
+  m->index = OP_map_index;
+  m->from = from;
+  m->to = to;
+  m->dim = dim;
+  m->map = imap;
+  m->name = name;
+  m->map_d = nullptr;  // copied to the device by the first loop that reads it
+  OP_map_list[OP_map_index++] = m;
+  return m;
+  API_END(nullptr)
+}
+
+op_dat op_decl_dat_core(op_set set, int dim, char const *type, int size, char *data,
+                        char const *name) {
+  API_BEGIN
+  if (!set) fail("op_decl_dat error -- invalid set for data: %s", name);
+  if (dim <= 0) fail("op_decl_dat error -- negative/zero dimension for data: %s", name);
+  grow(&OP_dat_list, &OP_dat_max, OP_dat_index);
+  op_dat d = (op_dat)malloc(sizeof(op_dat_core));
+  d->index = OP_dat_index;
+  d->set = set;
+  d->dim = dim;
+  d->data = data;
+  d->data_d = nullptr;
+  d->name = name;
+  d->type = type;
+  d->size = dim * size;
+  OP_dat_list[OP_dat_index++] = d;
+  return d;
+  API_END(nullptr)
+}
+
+op_dat op2_decl_dat_char(op_set set, int dim, char const *type, int size, char *data,
+                         char const *name) {
+  API_BEGIN
+  op_dat d = op_decl_dat_core(set, dim, type, size, data, name);
+  if (!d) return nullptr;
+  size_t bytes = (size_t)d->size * set->size;
+  d->data_d = (char *)dev_alloc(bytes);
+  if (data)
+    h2d(d->data_d, data, bytes);
+  else
+    OP2_HIP_CHECK(hipMemsetAsync(d->data_d, 0, bytes, rt().stream));
+  return d;
+  API_END(nullptr)
+}
+
+void op2_decl_const_char(int dim, char const *type, int size, char *data, char const *name) {
+  API_BEGIN
+  if (!name || !*name) fail("op_decl_const error -- constant has no name");
+  if (dim <= 0 || size <= 0 || !data) fail("op_decl_const error -- invalid constant %s", name);
+  Const c;
+  c.type = type ? type : "";
+  c.dim = dim;
+  c.size = size;
+  c.bytes.assign(data, data + (size_t)dim * size);
+  rt().consts[name] = c;
+  API_END()
+}
+
+void op2_fetch_data(op_dat dat) {
+  API_BEGIN
+  if (!dat || !dat->data_d || !dat->data) fail("op_fetch_data error -- dat has no device copy");
+  d2h(dat->data, dat->data_d, (size_t)dat->size * dat->set->size);
+  API_END()
+}
+
+static void arg_err(const char *msg, int m, const char *name) {
+  fail("error: arg %d in kernel \"%s\"\n%s ", m, name, msg);
+}
+
+void op_arg_check(op_set set, int m, op_arg arg, int *ninds, char const *name) {
+  API_BEGIN
+  if (arg.argtype == OP_ARG_DAT) {
+    if (!set) arg_err("invalid set", m, name);
+    if (!arg.dat) arg_err("invalid dataset", m, name);
+    if (!arg.map && arg.dat->set != set) arg_err("dataset set does not match loop set", m, name);
+    if (arg.map && (arg.map->from != set || arg.map->to != arg.dat->set))
+      arg_err("mapping error", m, name);
+    if ((!arg.map && arg.idx != -1) || (arg.map && (arg.idx < 0 || arg.idx >= arg.map->dim)))
+      arg_err("invalid index", m, name);
+    if (arg.dat->dim != arg.dim) arg_err("dataset dim does not match declared dim", m, name);
+    if (strcmp(arg.dat->type, arg.type)) arg_err("dataset type does not match declared type", m, name);
+    if (arg.idx >= 0) (*ninds)++;
+  }
+  if (arg.argtype == OP_ARG_GBL) {
+    if (!strcmp(arg.type, "error")) arg_err("datatype does not match declared type", m, name);
+    if (arg.dim <= 0) arg_err("dimension should be strictly positive", m, name);
+    if (!arg.data) arg_err("NULL pointer for global data", m, name);
+  }
+  API_END()
+}
+
+op_arg op_arg_dat(op_dat dat, int idx, op_map map, int dim, char const *type, op_access acc) {
+  op_arg a;
+  memset(&a, 0, sizeof a);
+  a.argtype = OP_ARG_DAT;
+  a.dat = dat;
+  a.map = map;
+  a.dim = dim;
+  a.idx = idx;
+  if (dat) {
+    a.size = dat->size;
+    a.data = dat->data;
+    a.data_d = dat->data_d;
+  }
+  a.type = type;
+  a.acc = acc;
+  return a;
+}
+
+op_arg op_arg_gbl_core(char *data, int dim, char const *type, op_access acc) {
+  op_arg a;
+  memset(&a, 0, sizeof a);
+  a.argtype = OP_ARG_GBL;
+  a.dat = nullptr;
+  a.map = nullptr;
+  a.dim = dim;
+  a.idx = -1;
+  a.size = 0;
+  a.data = data;
+  a.type = type;
+  a.acc = acc;
+  return a;
+}
+
+void op_diagnostic_output(void) {
+  if (OP_diags <= 1) return;
+  printf("\n  OP diagnostic output\n");
+  printf(  "  --------------------\n");
+  printf("\n       set       size\n");
+  printf(  "  -------------------\n");
+  for (int n = 0; n < OP_set_index; n++)
+    printf("%10s %10d\n", OP_set_list[n]->name, OP_set_list[n]->size);
+  printf("\n       map        dim       from         to\n");
+  printf(  "  -----------------------------------------\n");
+  for (int n = 0; n < OP_map_index; n++)
+    printf("%10s %10d %10s %10s\n", OP_map_list[n]->name, OP_map_list[n]->dim,
+           OP_map_list[n]->from->name, OP_map_list[n]->to->name);
+  printf("\n       dat        dim        set\n");
+  printf(  "  ------------------------------\n");
+  for (int n = 0; n < OP_dat_index; n++)
+    printf("%10s %10d %10s\n", OP_dat_list[n]->name, OP_dat_list[n]->dim,
+           OP_dat_list[n]->set->name);
+  printf("\n");
+}
+
+void op_timing_output(void) {
+  API_BEGIN
+  timing_drain();
+  Runtime &r = rt();
+  if (r.timings.empty()) return;
+  printf("\n  count     time     GB/s     kernel name ");
+  printf("\n -------------------------------------------- \n");
+  for (auto &t : r.timings) {
+    if (!t.count) continue;
+    double gbs = t.seconds > 0 ? t.bytes_per_call * t.count / (1e9 * t.seconds) : 0.0;
+    printf(" %6d  %8.4f %8.2f     %s \n", t.count, t.seconds, gbs, t.name.c_str());
+  }
+  API_END()
+}
+
+void op_timers(double *cpu, double *et) {
+  (void)cpu;
+  struct timeval t;
+  gettimeofday(&t, (struct timezone *)0);
+  *et = t.tv_sec + t.tv_usec * 1.0e-6;
+}
+
+void op_timing_realloc(int kernel) {
+  API_BEGIN
+  if (kernel >= OP_kern_max) {
+    int nmax = kernel + 10;
+    op_kernel *k = (op_kernel *)realloc(OP_kernels, nmax * sizeof(op_kernel));
+    if (!k) fail(" op_timing_realloc error ");
+    OP_kernels = k;
+    for (int n = OP_kern_max; n < nmax; n++) {
+      OP_kernels[n].count = 0;
+      OP_kernels[n].time = 0.0f;
+      OP_kernels[n].transfer = 0.0f;
+      OP_kernels[n].transfer2 = 0.0f;
+      OP_kernels[n].name = "unused";
+    }
+    OP_kern_max = nmax;
+  }
+  API_END()
+}
+
+void op_exit_core(void) {
+  API_BEGIN
+  clear_all();
+  API_END()
+}
+
+void op2_exit(void) {
+  API_BEGIN
+  clear_all();
+  API_END()
+}
+
+int op2_set_error_mode(int mode) {
+  int old = rt().error_mode;
+  rt().error_mode = mode;
+  return old;
+}
+
+char const *op2_last_error(void) { return rt().last_error.c_str(); }
+
+void op2_clear_error(void) { rt().last_error.clear(); }
+
+int op2_set_loop_config(char const *name, int strategy, int part_size, int block_size) {
+  API_BEGIN
+  if (strategy < 0 || strategy > OP2_STRATEGY_ATOMIC) fail("op2: unknown strategy %d", strategy);
+  if (part_size < 0 || part_size > 65535) fail("op2: part size %d out of range", part_size);
+  if (block_size != 0 && (block_size < 64 || block_size > 1024 || block_size % 64))
+    fail("op2: block size %d must be a multiple of 64 in [64, 1024]", block_size);
+  LoopConfig c{strategy, part_size, block_size};
+  if (name)
+    rt().loop_configs[name] = c;
+  else {
+    rt().default_config = c;
+    rt().loop_configs.clear();
+  }
+  return 0;
+  API_END(-1)
+}
+
+int op2_loop_stats(char const *name, int *count, double *seconds, double *bytes_per_call) {
+  API_BEGIN
+  timing_drain();
+  for (auto &t : rt().timings)
+    if (t.name == name) {
+      if (count) *count = t.count;
+      if (seconds) *seconds = t.seconds;
+      if (bytes_per_call) *bytes_per_call = t.bytes_per_call;
+      return 0;
+    }
+  return -1;
+  API_END(-1)
+}
+
+void op2_timing_reset(void) {
+  API_BEGIN
+  timing_drain();
+  for (auto &t : rt().timings) {
+    t.count = 0;
+    t.seconds = 0;
+  }
+  API_END()
+}
+
+void *op2_stream(void) {
+  API_BEGIN
+  ensure_device();
+  return (void *)rt().stream;
+  API_END(nullptr)
+}
+
+int op2_sync(void) {
+  API_BEGIN
+  ensure_device();
+  OP2_HIP_CHECK(hipStreamSynchronize(rt().stream));
+  return 0;
+  API_END(-1)
+}
+
+void *op2_dat_device_ptr(op_dat dat) { return dat ? (void *)dat->data_d : nullptr; }
+
+int op2_par_loop(char const *name, op_set set, int nargs, op_arg *args) {
+  API_BEGIN
+  return par_loop(name, set, nargs, args);
+  API_END(-1)
+}
+
+}  // extern "C"

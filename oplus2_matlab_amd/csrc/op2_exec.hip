@@ -1,0 +1,730 @@
+// Loop executor: recognises each op_par_loop by name and argument signature
+// and runs its hand-written gfx950 kernel.
+//
+// Loops (reference user kernel / generated CUDA stub they replace):
+//   save_soln  airfoil/save_soln.h,  airfoil/save_soln_kernel.cu:13-117
+//   adt_calc   airfoil/adt_calc.h,   airfoil/adt_calc_kernel.cu:13-164
+//   res_calc   airfoil/res_calc.h,   airfoil/res_calc_kernel.cu:13-245
+//   bres_calc  airfoil/bres_calc.h,  airfoil/bres_calc_kernel.cu:13-231
+//   update     airfoil/update.h,     airfoil/update_kernel.cu:13-182
+//   res        jac1/res.h,           jac1/res_kernel.cu:13-217
+//   update     jac1/update.h,        jac1/update_kernel.cu:13-153
+// Dats of type "double" run the FP64 kernels.  Dats of type "float" run the
+// precision form: float state and flux arithmetic, with update's
+// qold - res/adt and every global reduction carried in double.
+#include <algorithm>
+#include <cstring>
+
+#include "op2_internal.h"
+#include "op2_kernels.h"
+
+namespace op2 {
+
+constexpr int NT = 256;  // threads per workgroup (4 waves)
+
+// ============================================================================
+// kernels
+// ============================================================================
+
+// save_soln: qold = q, as 16-byte chunks
+__global__ __launch_bounds__(NT) void k_copy16(const uint4 *__restrict__ src,
+                                               uint4 *__restrict__ dst, long long n) {
+  long long i = (long long)blockIdx.x * NT + threadIdx.x;
+  const long long stride = (long long)gridDim.x * NT;
+  for (; i < n; i += stride) dst[i] = src[i];
+}
+
+// adt_calc: one cell per thread, nodes gathered through pcell
+template <typename T>
+__global__ __launch_bounds__(NT) void k_adt_calc(const int4 *__restrict__ pcell,
+                                                 const T *__restrict__ x,
+                                                 const T *__restrict__ q, T *__restrict__ adt,
+                                                 int ncell, AirfoilConsts<T> k) {
+  const int c = blockIdx.x * NT + threadIdx.x;
+  if (c >= ncell) return;
+  const int4 nd = pcell[c];
+  T x1[2], x2[2], x3[2], x4[2], qc[4];
+  Vec<T>::load2(x + 2 * (size_t)nd.x, x1);
+  Vec<T>::load2(x + 2 * (size_t)nd.y, x2);
+  Vec<T>::load2(x + 2 * (size_t)nd.z, x3);
+  Vec<T>::load2(x + 2 * (size_t)nd.w, x4);
+  Vec<T>::load4(q + 4 * (size_t)c, qc);
+  adt[c] = adt_of(x1, x2, x3, x4, qc, k);
+}
+
+// Staged edge loop (res_calc).  One workgroup per plan block:
+//  1. stage x of the block's nodes and q, adt of its cells in LDS;
+//  2. compute the flux of each edge (EPT edges per thread) into registers;
+//  3. reuse the q area as the block's res accumulator:
+//     COLOURED - thread colours in turn, one barrier per colour;
+//     otherwise - LDS atomic adds;
+//  4. add the accumulator into res: plain read-modify-write when blocks of
+//     one launch share no cell (block colours), global atomics otherwise.
+template <typename T, int EPT, bool COLOURED>
+__global__ __launch_bounds__(NT) void k_res_calc(
+    const int *__restrict__ blk_off, const int *__restrict__ blkmap, int blk_first, int nblk,
+    const int *__restrict__ node_off, const int *__restrict__ node_list,
+    const int *__restrict__ cell_off, const int *__restrict__ cell_list,
+    const ushort4 *__restrict__ loc, const int *__restrict__ col_off,
+    const int *__restrict__ col_base, const int *__restrict__ blk_ncol, const T *__restrict__ x,
+    const T *__restrict__ q, const T *__restrict__ adt, T *__restrict__ res,
+    AirfoilConsts<T> k) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int b = blkmap[blk_first + xcd_block(blockIdx.x, nblk)];
+  const int e0 = blk_off[b], ne = blk_off[b + 1] - e0;
+  const int n0 = node_off[b], nn = node_off[b + 1] - n0;
+  const int c0 = cell_off[b], nc = cell_off[b + 1] - c0;
+  T *xs = reinterpret_cast<T *>(smem);                   // [nn][2]
+  T *qs = xs + 2 * ((nn + 1) & ~1);                      // [nc][4], 16-byte aligned
+  T *as = qs + 4 * nc;                                   // [nc]
+  const int tid = threadIdx.x;
+
+  for (int i = tid; i < nn; i += NT) {
+    T v[2];
+    Vec<T>::load2(x + 2 * (size_t)node_list[n0 + i], v);
+    Vec<T>::store2(xs + 2 * i, v);
+  }
+  for (int i = tid; i < nc; i += NT) {
+    const int g = cell_list[c0 + i];
+    T v[4];
+    Vec<T>::load4(q + 4 * (size_t)g, v);
+    Vec<T>::store4(qs + 4 * i, v);
+    as[i] = adt[g];
+  }
+  __syncthreads();
+
+  T f[EPT][4];
+  int l1[EPT], l2[EPT];
+#pragma unroll
+  for (int j = 0; j < EPT; j++) {
+    const int e = tid + j * NT;
+    l1[j] = -1;
+    if (e < ne) {
+      const ushort4 L = loc[e0 + e];
+      T x1[2], x2[2], q1[4], q2[4];
+      Vec<T>::load2(xs + 2 * L.x, x1);
+      Vec<T>::load2(xs + 2 * L.y, x2);
+      Vec<T>::load4(qs + 4 * L.z, q1);
+      Vec<T>::load4(qs + 4 * L.w, q2);
+      edge_flux(x1, x2, q1, q2, as[L.z], as[L.w], k, f[j]);
+      l1[j] = L.z;
+      l2[j] = L.w;
+    }
+  }
+  __syncthreads();
+  T *rs = qs;
+  for (int i = tid; i < 4 * nc; i += NT) rs[i] = T(0);
+  __syncthreads();
+
+  if (COLOURED) {
+    const int cb = col_base[b], ncol = blk_ncol[b];
+    for (int c = 0; c < ncol; c++) {
+      const int cs = col_off[cb + c] - e0, ce = col_off[cb + c + 1] - e0;
+#pragma unroll
+      for (int j = 0; j < EPT; j++) {
+        const int e = tid + j * NT;
+        if (e >= cs && e < ce) {
+#pragma unroll
+          for (int d = 0; d < 4; d++) rs[4 * l1[j] + d] += f[j][d];
+#pragma unroll
+          for (int d = 0; d < 4; d++) rs[4 * l2[j] + d] -= f[j][d];
+        }
+      }
+      __syncthreads();
+    }
+    for (int i = tid; i < nc; i += NT) {
+      const int g = cell_list[c0 + i];
+      T r[4], a[4];
+      Vec<T>::load4(res + 4 * (size_t)g, r);
+      Vec<T>::load4(rs + 4 * i, a);
+#pragma unroll
+      for (int d = 0; d < 4; d++) r[d] += a[d];
+      Vec<T>::store4(res + 4 * (size_t)g, r);
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < EPT; j++)
+      if (l1[j] >= 0) {
+#pragma unroll
+        for (int d = 0; d < 4; d++) atomicAdd(rs + 4 * l1[j] + d, f[j][d]);
+#pragma unroll
+        for (int d = 0; d < 4; d++) atomicAdd(rs + 4 * l2[j] + d, -f[j][d]);
+      }
+    __syncthreads();
+    for (int i = tid; i < 4 * nc; i += NT)
+      atomicAdd(res + 4 * (size_t)cell_list[c0 + (i >> 2)] + (i & 3), rs[i]);
+  }
+}
+
+// Staged boundary-edge loop (bres_calc): as k_res_calc with one increment per
+// element and the direct `bound` read through the plan's element order.
+template <typename T, bool COLOURED>
+__global__ __launch_bounds__(NT) void k_bres_calc(
+    const int *__restrict__ blk_off, const int *__restrict__ blkmap, int blk_first, int nblk,
+    const int *__restrict__ perm, const int *__restrict__ node_off,
+    const int *__restrict__ node_list, const int *__restrict__ cell_off,
+    const int *__restrict__ cell_list, const ushort4 *__restrict__ loc,
+    const int *__restrict__ col_off, const int *__restrict__ col_base,
+    const int *__restrict__ blk_ncol, const T *__restrict__ x, const T *__restrict__ q,
+    const T *__restrict__ adt, T *__restrict__ res, const int *__restrict__ bound,
+    AirfoilConsts<T> k) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int b = blkmap[blk_first + xcd_block(blockIdx.x, nblk)];
+  const int e0 = blk_off[b], ne = blk_off[b + 1] - e0;
+  const int n0 = node_off[b], nn = node_off[b + 1] - n0;
+  const int c0 = cell_off[b], nc = cell_off[b + 1] - c0;
+  T *xs = reinterpret_cast<T *>(smem);
+  T *qs = xs + 2 * ((nn + 1) & ~1);
+  T *as = qs + 4 * nc;
+  T *rs = as + ((nc + 3) & ~3);
+  const int tid = threadIdx.x;
+  for (int i = tid; i < nn; i += NT) {
+    T v[2];
+    Vec<T>::load2(x + 2 * (size_t)node_list[n0 + i], v);
+    Vec<T>::store2(xs + 2 * i, v);
+  }
+  for (int i = tid; i < nc; i += NT) {
+    const int g = cell_list[c0 + i];
+    T v[4];
+    Vec<T>::load4(q + 4 * (size_t)g, v);
+    Vec<T>::store4(qs + 4 * i, v);
+    as[i] = adt[g];
+  }
+  for (int i = tid; i < 4 * nc; i += NT) rs[i] = T(0);
+  __syncthreads();
+  const int cb = COLOURED ? col_base[b] : 0;
+  const int ncol = COLOURED ? blk_ncol[b] : 1;
+  for (int c = 0; c < ncol; c++) {
+    const int cs = COLOURED ? col_off[cb + c] - e0 : 0;
+    const int ce = COLOURED ? col_off[cb + c + 1] - e0 : ne;
+    for (int e = cs + tid; e < ce; e += NT) {
+      const ushort4 L = loc[e0 + e];
+      const int el = perm ? perm[e0 + e] : e0 + e;
+      T x1[2], x2[2], q1[4], f[4];
+      Vec<T>::load2(xs + 2 * L.x, x1);
+      Vec<T>::load2(xs + 2 * L.y, x2);
+      Vec<T>::load4(qs + 4 * L.z, q1);
+      boundary_flux(x1, x2, q1, as[L.z], bound[el], k, f);
+      if (COLOURED) {
+#pragma unroll
+        for (int d = 0; d < 4; d++) rs[4 * L.z + d] += f[d];
+      } else {
+#pragma unroll
+        for (int d = 0; d < 4; d++) atomicAdd(rs + 4 * L.z + d, f[d]);
+      }
+    }
+    __syncthreads();
+  }
+  if (COLOURED) {
+    for (int i = tid; i < nc; i += NT) {
+      const int g = cell_list[c0 + i];
+      T r[4], a[4];
+      Vec<T>::load4(res + 4 * (size_t)g, r);
+      Vec<T>::load4(rs + 4 * i, a);
+#pragma unroll
+      for (int d = 0; d < 4; d++) r[d] += a[d];
+      Vec<T>::store4(res + 4 * (size_t)g, r);
+    }
+  } else {
+    for (int i = tid; i < 4 * nc; i += NT)
+      atomicAdd(res + 4 * (size_t)cell_list[c0 + (i >> 2)] + (i & 3), rs[i]);
+  }
+}
+
+// update: q = qold - res/adt, res = 0, rms += sum of squared moves; the
+// move and the new q are formed in double and q rounded once to T.
+template <typename T>
+__global__ __launch_bounds__(NT) void k_update(const T *__restrict__ qold, T *__restrict__ q,
+                                               T *__restrict__ res, const T *__restrict__ adt,
+                                               int ncell, double *partials, unsigned *ticket,
+                                               double *out) {
+  __shared__ double lds[NT / kWave];
+  __shared__ bool last;
+  double acc = 0.0;
+  for (int c = blockIdx.x * NT + threadIdx.x; c < ncell; c += gridDim.x * NT) {
+    T qo[4], r[4], qn[4];
+    Vec<T>::load4(qold + 4 * (size_t)c, qo);
+    Vec<T>::load4(res + 4 * (size_t)c, r);
+    const double adti = 1.0 / (double)adt[c];
+#pragma unroll
+    for (int d = 0; d < 4; d++) {
+      const double del = adti * (double)r[d];
+      qn[d] = (T)((double)qo[d] - del);
+      acc += del * del;
+      r[d] = T(0);
+    }
+    Vec<T>::store4(q + 4 * (size_t)c, qn);
+    Vec<T>::store4(res + 4 * (size_t)c, r);
+  }
+  grid_reduce<kRedSum, NT>(acc, partials, ticket, out, lds, &last);
+}
+
+// Staged Jacobi edge loop (jac1 res): du[pp0] += beta * A * u[pp1]; one map,
+// slots (pp, 1) for u and (pp, 0) for du.
+template <typename T, bool COLOURED>
+__global__ __launch_bounds__(NT) void k_jac_res(
+    const int *__restrict__ blk_off, const int *__restrict__ blkmap, int blk_first, int nblk,
+    const int *__restrict__ perm, const int *__restrict__ node_off,
+    const int *__restrict__ node_list, const ushort2 *__restrict__ loc,
+    const int *__restrict__ col_off, const int *__restrict__ col_base,
+    const int *__restrict__ blk_ncol, const T *__restrict__ A, const T *__restrict__ u,
+    T *__restrict__ du, T beta) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int b = blkmap[blk_first + xcd_block(blockIdx.x, nblk)];
+  const int e0 = blk_off[b], ne = blk_off[b + 1] - e0;
+  const int n0 = node_off[b], nn = node_off[b + 1] - n0;
+  T *us = reinterpret_cast<T *>(smem);
+  T *ds = us + nn;
+  const int tid = threadIdx.x;
+  for (int i = tid; i < nn; i += NT) {
+    us[i] = u[node_list[n0 + i]];
+    ds[i] = T(0);
+  }
+  __syncthreads();
+  const int cb = COLOURED ? col_base[b] : 0;
+  const int ncol = COLOURED ? blk_ncol[b] : 1;
+  for (int c = 0; c < ncol; c++) {
+    const int cs = COLOURED ? col_off[cb + c] - e0 : 0;
+    const int ce = COLOURED ? col_off[cb + c + 1] - e0 : ne;
+    for (int e = cs + tid; e < ce; e += NT) {
+      const ushort2 L = loc[e0 + e];
+      const int el = perm ? perm[e0 + e] : e0 + e;
+      const T v = beta * A[el] * us[L.x];
+      if (COLOURED)
+        ds[L.y] += v;
+      else
+        atomicAdd(ds + L.y, v);
+    }
+    __syncthreads();
+  }
+  for (int i = tid; i < nn; i += NT) {
+    const int g = node_list[n0 + i];
+    if (COLOURED)
+      du[g] += ds[i];
+    else
+      atomicAdd(du + g, ds[i]);
+  }
+}
+
+// jac1 update: u += du + alpha r, du = 0, u_sum += u^2, u_max = max(u_max, u)
+template <typename T>
+__global__ __launch_bounds__(NT) void k_jac_update(const T *__restrict__ r, T *__restrict__ du,
+                                                   T *__restrict__ u, int n, T alpha,
+                                                   double *partials, unsigned *tickets,
+                                                   double *out) {
+  __shared__ double lds[NT / kWave];
+  __shared__ bool last;
+  double s = 0.0, m = red_identity<kRedMax>();
+  for (int i = blockIdx.x * NT + threadIdx.x; i < n; i += gridDim.x * NT) {
+    T ui = u[i];
+    ui += du[i] + alpha * r[i];
+    u[i] = ui;
+    du[i] = T(0);
+    s += (double)ui * (double)ui;
+    m = (m > (double)ui) ? m : (double)ui;
+  }
+  grid_reduce<kRedSum, NT>(s, partials, tickets, out, lds, &last);
+  __syncthreads();
+  grid_reduce<kRedMax, NT>(m, partials + gridDim.x, tickets + 1, out + 1, lds, &last);
+}
+
+// ============================================================================
+// host side
+// ============================================================================
+
+namespace {
+
+bool is_type(const op_arg &a, const char *t) { return a.type && !strcmp(a.type, t); }
+
+bool direct(const op_arg &a, int dim, op_access acc, op_set set) {
+  return a.argtype == OP_ARG_DAT && !a.map && a.idx == -1 && a.dim == dim && a.acc == acc &&
+         a.dat && a.dat->set == set && a.dat->dim == dim;
+}
+
+bool via(const op_arg &a, op_map m, int idx, int dim, op_access acc) {
+  return a.argtype == OP_ARG_DAT && a.map == m && m && a.idx == idx && a.dim == dim &&
+         a.acc == acc && a.dat && a.dat->dim == dim && a.dat->set == m->to;
+}
+
+bool gbl(const op_arg &a, int dim, op_access acc) {
+  return a.argtype == OP_ARG_GBL && a.dim == dim && a.acc == acc && a.data;
+}
+
+// element type of the dat arguments: 8 (double), 4 (float) or 0 (mixed /
+// other)
+int real_bytes(int nargs, const op_arg *args) {
+  int rb = -1;
+  for (int i = 0; i < nargs; i++) {
+    if (args[i].argtype != OP_ARG_DAT) continue;
+    if (!strcmp(args[i].type, "int")) continue;
+    int t = is_type(args[i], "double") ? 8 : is_type(args[i], "float") ? 4 : 0;
+    if (rb >= 0 && t != rb) return 0;
+    rb = t;
+  }
+  return rb < 0 ? 0 : rb;
+}
+
+// Algorithmic bytes of one call: each touched dat once per direction (read
+// for READ/RW/INC, write for WRITE/RW/INC), each map array once; an
+// indirect dat counts min(target set size, slots x loop set size) elements.
+double loop_bytes(op_set set, int nargs, const op_arg *args) {
+  double bytes = 0;
+  std::vector<op_dat> dats;
+  std::vector<op_map> maps;
+  for (int i = 0; i < nargs; i++) {
+    const op_arg &a = args[i];
+    if (a.argtype != OP_ARG_DAT) continue;
+    if (a.map && std::find(maps.begin(), maps.end(), a.map) == maps.end()) {
+      maps.push_back(a.map);
+      bytes += 4.0 * a.map->dim * a.map->from->size;
+    }
+    if (std::find(dats.begin(), dats.end(), a.dat) != dats.end()) continue;
+    dats.push_back(a.dat);
+    bool rd = false, wr = false;
+    std::vector<std::pair<op_map, int>> slots;
+    for (int j = 0; j < nargs; j++) {
+      const op_arg &b = args[j];
+      if (b.argtype != OP_ARG_DAT || b.dat != a.dat) continue;
+      rd |= b.acc == OP_READ || b.acc == OP_RW || b.acc == OP_INC;
+      wr |= b.acc == OP_WRITE || b.acc == OP_RW || b.acc == OP_INC;
+      std::pair<op_map, int> s{b.map, b.idx};
+      if (std::find(slots.begin(), slots.end(), s) == slots.end()) slots.push_back(s);
+    }
+    double elems = a.map ? std::min<double>(a.dat->set->size, (double)slots.size() * set->size)
+                         : set->size;
+    bytes += elems * a.dat->size * ((rd ? 1 : 0) + (wr ? 1 : 0));
+  }
+  return bytes;
+}
+
+template <typename T>
+AirfoilConsts<T> airfoil_consts() {
+  AirfoilConsts<T> k;
+  k.gam = (T)get_const("gam").as_double(0);
+  k.gm1 = (T)get_const("gm1").as_double(0);
+  k.cfl = (T)get_const("cfl").as_double(0);
+  k.eps = (T)get_const("eps").as_double(0);
+  const Const &qi = get_const("qinf");
+  for (int i = 0; i < 4; i++) k.qinf[i] = (T)qi.as_double(i);
+  return k;
+}
+
+int grid_for(long long n, int cap) {
+  long long g = (n + NT - 1) / NT;
+  if (g < 1) g = 1;
+  return (int)std::min<long long>(g, cap);
+}
+
+void check_launch(const char *loop) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) fail("op2: kernel launch for loop %s failed: %s", loop, hipGetErrorString(e));
+}
+
+int strategy_of(const std::string &loop) {
+  int s = config_for(loop).strategy;
+  return s ? s : OP2_STRATEGY_ATOMIC;
+}
+
+int part_of(const std::string &loop, int dflt) {
+  int p = config_for(loop).part_size;
+  if (!p) p = OP_part_size;
+  return p ? p : dflt;
+}
+
+Plan *plan_for(const char *name, op_set set, int nargs, op_arg *args, int strategy, int part) {
+  PlanRequest req{name, set, nargs, args, strategy, part};
+  Plan *p = get_plan(req);
+  p->upload();
+  return p;
+}
+
+// reads `n` reduction results back to the host (waits for the stream)
+const double *fetch_results(const double *d_out, int n) {
+  Runtime &r = rt();
+  OP2_HIP_CHECK(hipMemcpyAsync(r.h_result, d_out, n * sizeof(double), hipMemcpyDeviceToHost,
+                               r.stream));
+  OP2_HIP_CHECK(hipStreamSynchronize(r.stream));
+  return r.h_result;
+}
+
+void gbl_inc(const op_arg &a, double v) {
+  if (is_type(a, "double"))
+    *(double *)a.data += v;
+  else if (is_type(a, "float"))
+    *(float *)a.data = (float)((double)*(float *)a.data + v);
+  else
+    fail("op2: OP_INC global of type %s is not supported", a.type);
+}
+
+void gbl_max(const op_arg &a, double v) {
+  if (is_type(a, "double")) {
+    double *p = (double *)a.data;
+    *p = (*p > v) ? *p : v;
+  } else if (is_type(a, "float")) {
+    float *p = (float *)a.data;
+    float fv = (float)v;
+    *p = (*p > fv) ? *p : fv;
+  } else {
+    fail("op2: OP_MAX global of type %s is not supported", a.type);
+  }
+}
+
+double gbl_read(const op_arg &a) {
+  if (is_type(a, "double")) return *(const double *)a.data;
+  if (is_type(a, "float")) return *(const float *)a.data;
+  fail("op2: OP_READ global of type %s is not supported", a.type);
+}
+
+// ---- per-loop launchers ------------------------------------------------------
+
+template <typename T>
+void run_save_soln(op_set set, op_arg *a) {
+  const long long chunks = (long long)set->size * 4 * sizeof(T) / 16;
+  hipLaunchKernelGGL(k_copy16, dim3(grid_for(chunks, 1 << 20)), dim3(NT), 0, rt().stream,
+                     (const uint4 *)a[0].dat->data_d, (uint4 *)a[1].dat->data_d, chunks);
+}
+
+template <typename T>
+void run_adt_calc(op_set set, op_arg *a) {
+  hipLaunchKernelGGL(k_adt_calc<T>, dim3(grid_for(set->size, 1 << 30)), dim3(NT), 0,
+                     rt().stream, (const int4 *)a[0].map->map_d, (const T *)a[0].dat->data_d,
+                     (const T *)a[4].dat->data_d, (T *)a[5].dat->data_d, set->size,
+                     airfoil_consts<T>());
+}
+
+template <typename T, int EPT>
+void launch_res_calc(Plan *p, op_arg *a, bool coloured) {
+  const PlanGroup &nodes = p->groups[p->slots[0].first];
+  const PlanGroup &cells = p->groups[p->slots[2].first];
+  const size_t lds = sizeof(T) * (2 * ((nodes.max_count + 1) & ~1) + 5 * (size_t)cells.max_count);
+  auto kern = coloured ? k_res_calc<T, EPT, true> : k_res_calc<T, EPT, false>;
+  if (lds > 64 * 1024)
+    OP2_HIP_CHECK(hipFuncSetAttribute((const void *)kern,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  AirfoilConsts<T> k = airfoil_consts<T>();
+  for (int c = 0; c < p->ncolors; c++) {
+    const int first = p->colblk_off[c], nb = p->colblk_off[c + 1] - first;
+    if (nb == 0) continue;
+    hipLaunchKernelGGL(kern, dim3(nb), dim3(NT), lds, rt().stream, p->d_blk_off, p->d_blkmap,
+                       first, nb, nodes.d_off, nodes.d_list, cells.d_off, cells.d_list,
+                       (const ushort4 *)p->d_loc, p->d_col_off, p->d_col_base, p->d_blk_ncol,
+                       (const T *)a[0].dat->data_d, (const T *)a[2].dat->data_d,
+                       (const T *)a[4].dat->data_d, (T *)a[6].dat->data_d, k);
+  }
+}
+
+template <typename T>
+void run_res_calc(const char *name, op_set set, int nargs, op_arg *a) {
+  const int strategy = strategy_of(name);
+  const int part = part_of(name, sizeof(T) == 8 ? 256 : 512);
+  if (part > 4 * NT) fail("op2: part size %d of %s exceeds %d", part, name, 4 * NT);
+  Plan *p = plan_for(name, set, nargs, a, strategy, part);
+  const bool col = strategy == OP2_STRATEGY_COLOURED;
+  if (part <= NT)
+    launch_res_calc<T, 1>(p, a, col);
+  else if (part <= 2 * NT)
+    launch_res_calc<T, 2>(p, a, col);
+  else
+    launch_res_calc<T, 4>(p, a, col);
+}
+
+template <typename T>
+void run_bres_calc(const char *name, op_set set, int nargs, op_arg *a) {
+  const int strategy = strategy_of(name);
+  Plan *p = plan_for(name, set, nargs, a, strategy, part_of(name, NT));
+  const bool col = strategy == OP2_STRATEGY_COLOURED;
+  const PlanGroup &nodes = p->groups[p->slots[0].first];
+  const PlanGroup &cells = p->groups[p->slots[2].first];
+  const size_t lds = sizeof(T) * (2 * ((nodes.max_count + 1) & ~1) + 4 * (size_t)cells.max_count +
+                                  ((cells.max_count + 3) & ~3) + 4 * (size_t)cells.max_count);
+  auto kern = col ? k_bres_calc<T, true> : k_bres_calc<T, false>;
+  if (lds > 64 * 1024)
+    OP2_HIP_CHECK(hipFuncSetAttribute((const void *)kern,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  AirfoilConsts<T> k = airfoil_consts<T>();
+  for (int c = 0; c < p->ncolors; c++) {
+    const int first = p->colblk_off[c], nb = p->colblk_off[c + 1] - first;
+    if (nb == 0) continue;
+    hipLaunchKernelGGL(kern, dim3(nb), dim3(NT), lds, rt().stream, p->d_blk_off, p->d_blkmap,
+                       first, nb, p->d_perm, nodes.d_off, nodes.d_list, cells.d_off,
+                       cells.d_list, (const ushort4 *)p->d_loc, p->d_col_off, p->d_col_base,
+                       p->d_blk_ncol, (const T *)a[0].dat->data_d, (const T *)a[2].dat->data_d,
+                       (const T *)a[3].dat->data_d, (T *)a[4].dat->data_d,
+                       (const int *)a[5].dat->data_d, k);
+  }
+}
+
+constexpr int kReduceBlocks = 2048;
+
+struct ReduceScratch {
+  double *partials;
+  unsigned *tickets;
+  double *out;
+};
+
+// layout: tickets (256 B), out (256 B), partials
+ReduceScratch reduce_scratch(int nslots) {
+  char *base = (char *)reduct_scratch(512 + sizeof(double) * kReduceBlocks * nslots);
+  return {(double *)(base + 512), (unsigned *)base, (double *)(base + 256)};
+}
+
+template <typename T>
+const double *run_update(op_set set, op_arg *a) {
+  ReduceScratch s = reduce_scratch(1);
+  const int grid = grid_for(set->size, kReduceBlocks);
+  hipLaunchKernelGGL(k_update<T>, dim3(grid), dim3(NT), 0, rt().stream,
+                     (const T *)a[0].dat->data_d, (T *)a[1].dat->data_d, (T *)a[2].dat->data_d,
+                     (const T *)a[3].dat->data_d, set->size, s.partials, s.tickets, s.out);
+  check_launch("update");
+  return s.out;
+}
+
+template <typename T>
+void run_jac_res(const char *name, op_set set, int nargs, op_arg *a) {
+  const int strategy = strategy_of(name);
+  Plan *p = plan_for(name, set, nargs, a, strategy, part_of(name, 1024));
+  const bool col = strategy == OP2_STRATEGY_COLOURED;
+  const PlanGroup &nodes = p->groups[0];
+  const size_t lds = sizeof(T) * 2 * (size_t)nodes.max_count;
+  auto kern = col ? k_jac_res<T, true> : k_jac_res<T, false>;
+  if (lds > 64 * 1024)
+    OP2_HIP_CHECK(hipFuncSetAttribute((const void *)kern,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  // slot order: first use is arg 1 (u via idx 1), then arg 2 (du via idx 0)
+  if (p->arg_slot[1] != 0 || p->arg_slot[2] != 1) fail("op2: unexpected slot layout for %s", name);
+  const T beta = (T)gbl_read(a[3]);
+  for (int c = 0; c < p->ncolors; c++) {
+    const int first = p->colblk_off[c], nb = p->colblk_off[c + 1] - first;
+    if (nb == 0) continue;
+    hipLaunchKernelGGL(kern, dim3(nb), dim3(NT), lds, rt().stream, p->d_blk_off, p->d_blkmap,
+                       first, nb, p->d_perm, nodes.d_off, nodes.d_list, (const ushort2 *)p->d_loc,
+                       p->d_col_off, p->d_col_base, p->d_blk_ncol, (const T *)a[0].dat->data_d,
+                       (const T *)a[1].dat->data_d, (T *)a[2].dat->data_d, beta);
+  }
+}
+
+template <typename T>
+const double *run_jac_update(op_set set, op_arg *a) {
+  ReduceScratch s = reduce_scratch(2);
+  const int grid = grid_for(set->size, kReduceBlocks);
+  const T alpha = (T)get_const("alpha").as_double(0);
+  hipLaunchKernelGGL(k_jac_update<T>, dim3(grid), dim3(NT), 0, rt().stream,
+                     (const T *)a[0].dat->data_d, (T *)a[1].dat->data_d, (T *)a[2].dat->data_d,
+                     set->size, alpha, s.partials, s.tickets, s.out);
+  check_launch("update");
+  return s.out;
+}
+
+enum class Loop { kUnknown, kSaveSoln, kAdtCalc, kResCalc, kBresCalc, kUpdate, kJacRes, kJacUpdate };
+
+Loop identify(const char *name, op_set set, int nargs, const op_arg *a) {
+  auto is = [&](const char *n) { return !strcmp(name, n); };
+  if (is("save_soln") && nargs == 2 && direct(a[0], 4, OP_READ, set) &&
+      direct(a[1], 4, OP_WRITE, set))
+    return Loop::kSaveSoln;
+  if (is("adt_calc") && nargs == 6 && a[0].map && a[0].map->dim == 4 && a[0].map->from == set) {
+    bool ok = true;
+    for (int i = 0; i < 4; i++) ok &= via(a[i], a[0].map, i, 2, OP_READ) && a[i].dat == a[0].dat;
+    if (ok && direct(a[4], 4, OP_READ, set) && direct(a[5], 1, OP_WRITE, set))
+      return Loop::kAdtCalc;
+  }
+  if (is("res_calc") && nargs == 8 && a[0].map && a[2].map && a[0].map->dim == 2 &&
+      a[2].map->dim == 2 && a[0].map->from == set && a[2].map->from == set) {
+    op_map pe = a[0].map, pc = a[2].map;
+    bool ok = via(a[0], pe, 0, 2, OP_READ) && via(a[1], pe, 1, 2, OP_READ) &&
+              a[1].dat == a[0].dat && via(a[2], pc, 0, 4, OP_READ) &&
+              via(a[3], pc, 1, 4, OP_READ) && a[3].dat == a[2].dat &&
+              via(a[4], pc, 0, 1, OP_READ) && via(a[5], pc, 1, 1, OP_READ) &&
+              a[5].dat == a[4].dat && via(a[6], pc, 0, 4, OP_INC) &&
+              via(a[7], pc, 1, 4, OP_INC) && a[7].dat == a[6].dat && pe != pc;
+    if (ok) return Loop::kResCalc;
+  }
+  if (is("bres_calc") && nargs == 6 && a[0].map && a[2].map && a[0].map->dim == 2 &&
+      a[2].map->dim == 1 && a[0].map->from == set && a[2].map->from == set) {
+    op_map pb = a[0].map, pc = a[2].map;
+    bool ok = via(a[0], pb, 0, 2, OP_READ) && via(a[1], pb, 1, 2, OP_READ) &&
+              a[1].dat == a[0].dat && via(a[2], pc, 0, 4, OP_READ) &&
+              via(a[3], pc, 0, 1, OP_READ) && via(a[4], pc, 0, 4, OP_INC) &&
+              direct(a[5], 1, OP_READ, set) && is_type(a[5], "int") && pb != pc;
+    if (ok) return Loop::kBresCalc;
+  }
+  if (is("update") && nargs == 5 && direct(a[0], 4, OP_READ, set) &&
+      direct(a[1], 4, OP_WRITE, set) && direct(a[2], 4, OP_RW, set) &&
+      direct(a[3], 1, OP_READ, set) && gbl(a[4], 1, OP_INC))
+    return Loop::kUpdate;
+  if (is("res") && nargs == 4 && direct(a[0], 1, OP_READ, set) && a[1].map &&
+      a[1].map->dim == 2 && via(a[1], a[1].map, 1, 1, OP_READ) &&
+      via(a[2], a[1].map, 0, 1, OP_INC) && gbl(a[3], 1, OP_READ))
+    return Loop::kJacRes;
+  if (is("update") && nargs == 5 && direct(a[0], 1, OP_READ, set) &&
+      direct(a[1], 1, OP_RW, set) && direct(a[2], 1, OP_INC, set) && gbl(a[3], 1, OP_INC) &&
+      gbl(a[4], 1, OP_MAX))
+    return Loop::kJacUpdate;
+  return Loop::kUnknown;
+}
+
+template <typename T>
+void dispatch(Loop l, const char *name, op_set set, int nargs, op_arg *a) {
+  switch (l) {
+    case Loop::kSaveSoln: run_save_soln<T>(set, a); break;
+    case Loop::kAdtCalc: run_adt_calc<T>(set, a); break;
+    case Loop::kResCalc: run_res_calc<T>(name, set, nargs, a); break;
+    case Loop::kBresCalc: run_bres_calc<T>(name, set, nargs, a); break;
+    case Loop::kJacRes: run_jac_res<T>(name, set, nargs, a); break;
+    default: break;
+  }
+}
+
+}  // namespace
+
+int par_loop(const char *name, op_set set, int nargs, op_arg *args) {
+  if (!name || !set || nargs <= 0 || !args) fail("op_par_loop error -- invalid arguments");
+  ensure_device();
+  if (OP_diags > 0) {
+    int ninds = 0;
+    for (int m = 0; m < nargs; m++) op_arg_check(set, m, args[m], &ninds, name);
+  }
+  for (int m = 0; m < nargs; m++) {
+    if (args[m].argtype != OP_ARG_DAT) continue;
+    if (!args[m].dat || !args[m].dat->data_d)
+      fail("op_par_loop error -- arg %d of %s has no device data (declare dats with op_decl_dat)",
+           m, name);
+    if (args[m].map) ensure_map_device(args[m].map);
+  }
+  const Loop l = identify(name, set, nargs, args);
+  if (l == Loop::kUnknown)
+    fail("op_par_loop error -- no device kernel for loop \"%s\" with this argument signature",
+         name);
+  const int rb = real_bytes(nargs, args);
+  if (rb != 4 && rb != 8)
+    fail("op_par_loop error -- loop \"%s\" needs all its real dats to be double or all float",
+         name);
+  if (OP_diags > 2) printf(" kernel routine on device: %s \n", name);
+
+  LoopTiming &t = timing_for(name, loop_bytes(set, nargs, args));
+  hipEvent_t start;
+  timing_begin(t, &start);
+  if (l == Loop::kUpdate) {
+    const double *d = rb == 8 ? run_update<double>(set, args) : run_update<float>(set, args);
+    timing_end(t, start);
+    gbl_inc(args[4], fetch_results(d, 1)[0]);
+    return 0;
+  }
+  if (l == Loop::kJacUpdate) {
+    const double *d = rb == 8 ? run_jac_update<double>(set, args) : run_jac_update<float>(set, args);
+    timing_end(t, start);
+    const double *h = fetch_results(d, 2);
+    gbl_inc(args[3], h[0]);
+    gbl_max(args[4], h[1]);
+    return 0;
+  }
+  if (rb == 8)
+    dispatch<double>(l, name, set, nargs, args);
+  else
+    dispatch<float>(l, name, set, nargs, args);
+  check_launch(name);
+  timing_end(t, start);
+  return 0;
+}
+
+}  // namespace op2

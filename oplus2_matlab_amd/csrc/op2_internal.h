@@ -1,0 +1,175 @@
+// Internal state of the MI355X OP2 runtime (not part of the public API).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "op2_hip.h"
+
+namespace op2 {
+
+// Raised by fail() when the error mode is OP2_ERRORS_RETURN; caught at the
+// C-ABI boundary.
+struct Error : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+// Reports an error: prints and exits (reference behaviour) or throws Error.
+[[noreturn]] void fail(const char *fmt, ...) __attribute__((format(printf, 1, 2)));
+
+#define OP2_HIP_CHECK(call)                                                     \
+  do {                                                                          \
+    hipError_t e_ = (call);                                                     \
+    if (e_ != hipSuccess)                                                       \
+      ::op2::fail("%s(%d): HIP error %s in %s", __FILE__, __LINE__,             \
+                  hipGetErrorString(e_), #call);                                \
+  } while (0)
+
+// A named constant from op_decl_const / op_decl_const2.
+struct Const {
+  std::string type;
+  int dim = 0, size = 0;
+  std::vector<char> bytes;
+  double as_double(int i) const;
+};
+
+struct LoopConfig {
+  int strategy = OP2_STRATEGY_AUTO;
+  int part_size = 0;
+  int block_size = 0;
+};
+
+// One staged map of a plan: the distinct target elements of each block.
+struct PlanGroup {
+  op_map map = nullptr;
+  std::vector<int> off;   // [nblocks+1] into list
+  std::vector<int> list;  // global target ids, ascending within a block
+  int max_count = 0;      // largest per-block count
+  int *d_off = nullptr, *d_list = nullptr;
+};
+
+// Execution plan of one indirect loop on one set (this runtime's design:
+// groups by map rather than by dat, packed 16-bit local indices per
+// (map, idx) slot, elements sorted by thread colour inside each block).
+struct Plan {
+  // key
+  std::string loop;
+  op_set set = nullptr;
+  int strategy = 0, part_size = 0;
+  std::vector<op_map> maps;
+  std::vector<int> idxs;
+  std::vector<int> accs;
+  std::vector<op_dat> dats;
+
+  // layout
+  int nblocks = 0, max_block = 0;
+  std::vector<int> blk_off;  // [nblocks+1] positions
+  std::vector<int> perm;     // position -> element (empty: identity)
+  bool need_perm = false;    // perm is uploaded for loops with direct dats
+  std::vector<PlanGroup> groups;
+  // slots: distinct (map, idx) of indirect args, in first-use order
+  std::vector<std::pair<int, int>> slots;  // (group, idx)
+  std::vector<int> arg_slot;               // per arg, -1 if direct / global
+  int slot_stride = 0;                     // 1, 2 or 4 (padded)
+  std::vector<uint16_t> loc;               // [set_size * slot_stride]
+  // thread colours (positions of colour k of block b are
+  // [col_off[col_base[b]+k], col_off[col_base[b]+k+1]))
+  std::vector<int> col_off, col_base, blk_ncol;
+  std::vector<int> thrcol;  // per position
+  // block colours
+  int ncolors = 0;
+  std::vector<int> blkcol;      // per block
+  std::vector<int> blkmap;      // blocks grouped by colour
+  std::vector<int> colblk_off;  // [ncolors+1]
+  long long verify_errors = -1;
+  double mean_thread_colours = 0;
+
+  // device copies
+  int *d_blk_off = nullptr, *d_perm = nullptr, *d_col_off = nullptr,
+      *d_col_base = nullptr, *d_blk_ncol = nullptr, *d_blkmap = nullptr;
+  uint16_t *d_loc = nullptr;
+
+  // the op_plan view returned by op_plan_core / op_plan_get
+  op_plan view{};
+  std::vector<int> view_offset, view_nelems, view_ncolblk, view_nthrcol;
+  int count = 0;
+
+  bool uploaded = false;
+  void upload();  // no-op once uploaded
+  void release();
+};
+
+// Per-loop timing record (the op_kernel table of op_lib_core.c:381-427).
+struct LoopTiming {
+  std::string name;
+  int count = 0;
+  double seconds = 0;
+  double bytes_per_call = 0;
+  struct Pending {
+    hipEvent_t start, stop;
+  };
+  std::vector<Pending> pending;
+};
+
+struct Runtime {
+  bool initialised = false;
+  bool device_ready = false;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int error_mode = OP2_ERRORS_EXIT;
+  std::string last_error;
+  std::map<std::string, Const> consts;
+  std::map<std::string, LoopConfig> loop_configs;
+  LoopConfig default_config;
+  std::vector<std::unique_ptr<Plan>> plans;
+  std::vector<LoopTiming> timings;
+  std::vector<hipEvent_t> event_pool;
+  // reduction scratch: per-block partials + ticket counter + result slots
+  void *d_reduct = nullptr;
+  size_t reduct_bytes = 0;
+  double *h_result = nullptr;  // pinned
+  bool timing_enabled = true;
+};
+
+Runtime &rt();
+void ensure_device();
+void *dev_alloc(size_t bytes);
+void dev_free(void *p);
+void h2d(void *dst, const void *src, size_t bytes);
+void d2h(void *dst, const void *src, size_t bytes);
+void ensure_map_device(op_map m);
+
+const Const &get_const(const char *name);
+LoopConfig config_for(const std::string &loop);
+
+// plan construction (op2_plan.cpp)
+struct PlanRequest {
+  std::string loop;
+  op_set set;
+  int nargs;
+  const op_arg *args;
+  int strategy;
+  int part_size;
+};
+Plan *get_plan(const PlanRequest &req);
+long long verify_plan(const Plan &p);
+
+// timing (op2_core.cpp)
+LoopTiming &timing_for(const std::string &name, double bytes_per_call);
+void timing_begin(LoopTiming &t, hipEvent_t *start);
+void timing_end(LoopTiming &t, hipEvent_t start);
+void timing_drain();
+
+// scratch for reductions: at least `bytes` of device memory, 256-aligned
+void *reduct_scratch(size_t bytes);
+
+// loop executor (op2_exec.hip)
+int par_loop(const char *name, op_set set, int nargs, op_arg *args);
+
+}  // namespace op2

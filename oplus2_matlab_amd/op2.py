@@ -1,0 +1,347 @@
+"""Python binding of the OP2 API over the C ABI of libop2hip.so (ctypes).
+
+Mirrors the reference API (common/op_lib.h, common/op_seq.h) name for name:
+op_init, op_decl_set, op_decl_map, op_decl_dat, op_decl_const /
+op_decl_const2, op_arg_dat, op_arg_gbl, op_par_loop, op_fetch_data,
+op_diagnostic_output, op_timing_output, op_exit, with OP_READ ... OP_MAX and
+OP_ID / OP_GBL.  Errors raise OP2Error instead of exiting the process.
+
+Every loop runs on the GPU through the runtime's gfx950 kernels; there is no
+host fallback.  The library must have been built (python __graft_entry__.py
+build, or make -C oplus2_matlab_amd/csrc).
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libop2hip.so")
+
+OP_READ, OP_WRITE, OP_RW, OP_INC, OP_MIN, OP_MAX = range(6)
+OP_ARG_GBL, OP_ARG_DAT = 0, 1
+OP_ID = None
+OP_GBL = None
+
+STRATEGY_AUTO, STRATEGY_COLOURED, STRATEGY_ATOMIC = 0, 1, 2
+ERRORS_EXIT, ERRORS_RETURN = 0, 1
+
+_NP_TYPE = {"double": np.float64, "float": np.float32, "int": np.int32,
+            "uint": np.uint32, "ll": np.int64, "ull": np.uint64, "bool": np.bool_}
+
+
+class OP2Error(RuntimeError):
+    pass
+
+
+class _SetCore(C.Structure):
+    _fields_ = [("index", C.c_int), ("size", C.c_int), ("name", C.c_char_p)]
+
+
+class _MapCore(C.Structure):
+    _fields_ = [("index", C.c_int), ("from_", C.POINTER(_SetCore)),
+                ("to", C.POINTER(_SetCore)), ("dim", C.c_int),
+                ("map", C.POINTER(C.c_int)), ("name", C.c_char_p),
+                ("map_d", C.c_void_p)]
+
+
+class _DatCore(C.Structure):
+    _fields_ = [("index", C.c_int), ("set", C.POINTER(_SetCore)), ("dim", C.c_int),
+                ("size", C.c_int), ("data", C.c_void_p), ("data_d", C.c_void_p),
+                ("type", C.c_char_p), ("name", C.c_char_p)]
+
+
+class OpArg(C.Structure):
+    """op_arg of common/op_lib_core.h:107-119 (passed by value)."""
+    _fields_ = [("index", C.c_int), ("dat", C.POINTER(_DatCore)),
+                ("map", C.POINTER(_MapCore)), ("dim", C.c_int), ("idx", C.c_int),
+                ("size", C.c_int), ("data", C.c_void_p), ("data_d", C.c_void_p),
+                ("type", C.c_char_p), ("acc", C.c_int), ("argtype", C.c_int)]
+
+
+class PlanSummary(C.Structure):
+    _fields_ = [("nblocks", C.c_int), ("ncolors", C.c_int), ("max_block", C.c_int),
+                ("max_staged", C.c_int), ("strategy", C.c_int), ("part_size", C.c_int),
+                ("mean_thread_colours", C.c_double), ("verify_errors", C.c_longlong)]
+
+
+# every extern "C" entry point of include/op2_hip.h: name -> (restype, argtypes)
+_P = C.c_void_p
+SIGNATURES = {
+    "op_init_core": (None, [C.c_int, C.POINTER(C.c_char_p), C.c_int]),
+    "op_decl_set": (_P, [C.c_int, C.c_char_p]),
+    "op_decl_map": (_P, [_P, _P, C.c_int, _P, C.c_char_p]),
+    "op_decl_dat_core": (_P, [_P, C.c_int, C.c_char_p, C.c_int, _P, C.c_char_p]),
+    "op_arg_check": (None, [_P, C.c_int, OpArg, C.POINTER(C.c_int), C.c_char_p]),
+    "op_arg_dat": (OpArg, [_P, C.c_int, _P, C.c_int, C.c_char_p, C.c_int]),
+    "op_arg_gbl_core": (OpArg, [_P, C.c_int, C.c_char_p, C.c_int]),
+    "op_diagnostic_output": (None, []),
+    "op_timing_output": (None, []),
+    "op_plan_core": (_P, [C.c_char_p, _P, C.c_int, C.c_int, C.POINTER(OpArg), C.c_int,
+                          C.POINTER(C.c_int)]),
+    "op_timers": (None, [C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "op_timing_realloc": (None, [C.c_int]),
+    "op_exit_core": (None, []),
+    "op2_init": (None, [C.c_int, C.POINTER(C.c_char_p), C.c_int]),
+    "op2_decl_dat_char": (_P, [_P, C.c_int, C.c_char_p, C.c_int, _P, C.c_char_p]),
+    "op2_decl_const_char": (None, [C.c_int, C.c_char_p, C.c_int, _P, C.c_char_p]),
+    "op2_fetch_data": (None, [_P]),
+    "op2_plan_get": (_P, [C.c_char_p, _P, C.c_int, C.c_int, C.POINTER(OpArg), C.c_int,
+                          C.POINTER(C.c_int)]),
+    "op2_exit": (None, []),
+    "op2_par_loop": (C.c_int, [C.c_char_p, _P, C.c_int, C.POINTER(OpArg)]),
+    "op2_set_error_mode": (C.c_int, [C.c_int]),
+    "op2_last_error": (C.c_char_p, []),
+    "op2_clear_error": (None, []),
+    "op2_set_loop_config": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int]),
+    "op2_loop_stats": (C.c_int, [C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_double),
+                                 C.POINTER(C.c_double)]),
+    "op2_timing_reset": (None, []),
+    "op2_stream": (_P, []),
+    "op2_sync": (C.c_int, []),
+    "op2_dat_device_ptr": (_P, [_P]),
+    "op2_plan_summary_get": (C.c_int, [C.c_char_p, _P, C.c_int, C.POINTER(OpArg), C.c_int,
+                                       C.c_int, C.POINTER(PlanSummary)]),
+    "op2_plan_export": (C.c_int, [C.c_char_p, _P, C.c_int, C.POINTER(OpArg), C.c_int, C.c_int,
+                                  _P, _P, _P, _P]),
+}
+
+_lib = None
+
+
+def lib():
+    """Loads libop2hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise OP2Error(f"{LIB_PATH} is missing: build it with "
+                           "`python __graft_entry__.py build` (make -C oplus2_matlab_amd/csrc)")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        L.op2_set_error_mode(ERRORS_RETURN)
+        _lib = L
+    return _lib
+
+
+def _call(fn, *args):
+    L = lib()
+    L.op2_clear_error()
+    out = getattr(L, fn)(*args)
+    msg = L.op2_last_error()
+    if msg:
+        raise OP2Error(msg.decode(errors="replace"))
+    return out
+
+
+def _b(s):
+    return None if s is None else s.encode()
+
+
+# ---------------------------------------------------------------------------
+# handles
+
+class Set:
+    def __init__(self, ptr, size, name):
+        self._p, self.size, self.name = ptr, size, name
+        self._name_b = _b(name)
+
+
+class Map:
+    def __init__(self, ptr, frm, to, dim, data, name):
+        self._p, self.from_set, self.to_set, self.dim = ptr, frm, to, dim
+        self.data, self.name = data, name
+
+
+class Dat:
+    def __init__(self, ptr, set_, dim, type_, data, name):
+        self._p, self.set, self.dim, self.type = ptr, set_, dim, type_
+        self.data, self.name = data, name
+
+    def device_ptr(self) -> int:
+        return lib().op2_dat_device_ptr(self._p)
+
+
+_keep = []  # names and arrays the library points at, released by op_exit
+
+
+def op_init(argv=None, diags: int = 0) -> None:
+    """op_init (common/op_lib.cu:128-143)."""
+    argv = [b"op2"] + [_b(a) for a in (argv or [])]
+    arr = (C.c_char_p * len(argv))(*argv)
+    _keep.append(arr)
+    _call("op2_init", len(argv), arr, diags)
+
+
+def op_decl_set(size: int, name: str) -> Set:
+    nb = _b(name)
+    _keep.append(nb)
+    return Set(_call("op_decl_set", size, nb), size, name)
+
+
+def op_decl_map(frm: Set, to: Set, dim: int, imap, name: str) -> Map:
+    a = np.ascontiguousarray(imap, dtype=np.int32).reshape(-1)
+    if a.size != dim * frm.size:
+        raise OP2Error(f"op_decl_map: {name} has {a.size} entries, expected {dim * frm.size}")
+    nb = _b(name)
+    _keep.extend([nb, a])
+    p = _call("op_decl_map", frm._p, to._p, dim, a.ctypes.data, nb)
+    return Map(p, frm, to, dim, a, name)
+
+
+def op_decl_dat(set_: Set, dim: int, type_: str, data, name: str) -> Dat:
+    """op_decl_dat<T> (common/op_lib.h:160-167): checks the element type
+    against `type_`, then copies the data to the device."""
+    want = _NP_TYPE.get(type_)
+    arr = np.asarray(data)
+    if want is None or arr.dtype != want:
+        raise OP2Error(f'incorrect type specified for dataset "{name}"')
+    if not arr.flags.c_contiguous:
+        raise OP2Error(f"op_decl_dat: {name} must be C-contiguous")
+    if arr.size != dim * set_.size:
+        raise OP2Error(f"op_decl_dat: {name} has {arr.size} values, expected {dim * set_.size}")
+    nb, tb = _b(name), _b(type_)
+    _keep.extend([nb, tb, arr])
+    p = _call("op2_decl_dat_char", set_._p, dim, tb, arr.dtype.itemsize, arr.ctypes.data, nb)
+    return Dat(p, set_, dim, type_, arr, name)
+
+
+def op_decl_dat_host(set_: Set, dim: int, type_: str, data, name: str) -> Dat:
+    """op_decl_dat_core (common/op_lib_core.c:171-207): registers the dat
+    without a device copy (enough for building and checking plans)."""
+    arr = np.ascontiguousarray(data, dtype=_NP_TYPE[type_])
+    nb, tb = _b(name), _b(type_)
+    _keep.extend([nb, tb, arr])
+    p = _call("op_decl_dat_core", set_._p, dim, tb, arr.dtype.itemsize, arr.ctypes.data, nb)
+    return Dat(p, set_, dim, type_, arr, name)
+
+
+def op_decl_const2(name: str, dim: int, type_: str, data) -> None:
+    want = _NP_TYPE.get(type_)
+    arr = np.ascontiguousarray(np.asarray(data, dtype=want)).reshape(-1)
+    if arr.size != dim:
+        raise OP2Error(f'constant "{name}" has {arr.size} values, expected {dim}')
+    _call("op2_decl_const_char", dim, _b(type_), arr.dtype.itemsize, arr.ctypes.data, _b(name))
+
+
+op_decl_const = op_decl_const2
+
+
+def op_arg_dat(dat: Dat, idx: int, map_: Map | None, dim: int, type_: str, acc: int) -> OpArg:
+    tb = _b(type_)
+    _keep.append(tb)
+    return lib().op_arg_dat(dat._p, idx, map_._p if map_ is not None else None, dim, tb, acc)
+
+
+def op_arg_gbl(data: np.ndarray, dim: int, type_: str, acc: int) -> OpArg:
+    """op_arg_gbl<T> (common/op_lib.h:184-190); `data` is a numpy array the
+    runtime reads (OP_READ) or updates in place (OP_INC / OP_MIN / OP_MAX)."""
+    if not isinstance(data, np.ndarray) or not data.flags.c_contiguous:
+        raise OP2Error("op_arg_gbl needs a C-contiguous numpy array")
+    want = _NP_TYPE.get(type_)
+    t = type_ if (want is not None and data.dtype == want) else "error"
+    tb = _b(t)
+    _keep.extend([tb, data])
+    return lib().op_arg_gbl_core(data.ctypes.data, dim, tb, acc)
+
+
+def op_par_loop(kernel, name: str, set_: Set, *args: OpArg) -> None:
+    """op_par_loop (common/op_seq.h): runs loop `name` on the device.  The
+    `kernel` argument names the user function, as in the reference; the
+    runtime selects its gfx950 kernel by loop name and argument signature."""
+    del kernel
+    arr = (OpArg * len(args))(*args)
+    rc = _call("op2_par_loop", _b(name), set_._p, len(args), arr)
+    if rc != 0:
+        raise OP2Error(lib().op2_last_error().decode())
+
+
+def op_fetch_data(dat: Dat) -> np.ndarray:
+    """op_fetch_data (common/op_lib.cu:116-121): copies the device data into
+    the array given to op_decl_dat and returns it."""
+    _call("op2_fetch_data", dat._p)
+    return dat.data
+
+
+def op_diagnostic_output() -> None:
+    lib().op_diagnostic_output()
+
+
+def op_timing_output() -> None:
+    _call("op_timing_output")
+
+
+def op_exit() -> None:
+    _call("op2_exit")
+    _keep.clear()
+
+
+# ---------------------------------------------------------------------------
+# runtime controls
+
+def set_loop_config(name: str | None, strategy: int = STRATEGY_AUTO, part_size: int = 0,
+                    block_size: int = 0) -> None:
+    rc = _call("op2_set_loop_config", _b(name), strategy, part_size, block_size)
+    if rc != 0:
+        raise OP2Error(lib().op2_last_error().decode())
+
+
+def loop_stats(name: str):
+    """(count, device seconds, algorithmic bytes per call) or None."""
+    c, s, b = C.c_int(), C.c_double(), C.c_double()
+    if _call("op2_loop_stats", _b(name), C.byref(c), C.byref(s), C.byref(b)) != 0:
+        return None
+    return c.value, s.value, b.value
+
+
+def timing_reset() -> None:
+    _call("op2_timing_reset")
+
+
+def sync() -> None:
+    _call("op2_sync")
+
+
+def stream() -> int:
+    return _call("op2_stream")
+
+
+def plan_summary(name: str, set_: Set, args, strategy: int = STRATEGY_COLOURED,
+                 part_size: int = 0) -> dict:
+    arr = (OpArg * len(args))(*args)
+    out = PlanSummary()
+    if _call("op2_plan_summary_get", _b(name), set_._p, len(args), arr, strategy, part_size,
+             C.byref(out)) != 0:
+        raise OP2Error(lib().op2_last_error().decode())
+    return {k: getattr(out, k) for k, _ in PlanSummary._fields_}
+
+
+def plan_export(name: str, set_: Set, args, strategy: int = STRATEGY_COLOURED,
+                part_size: int = 0) -> dict:
+    """The plan's element order, thread colours, block ranges and block
+    colours (built and verified on the host; no GPU needed)."""
+    arr = (OpArg * len(args))(*args)
+    nb = _call("op2_plan_export", _b(name), set_._p, len(args), arr, strategy, part_size,
+               None, None, None, None)
+    if nb < 0:
+        raise OP2Error(lib().op2_last_error().decode())
+    n = set_.size
+    perm = np.zeros(n, np.int32)
+    thr = np.zeros(n, np.int32)
+    off = np.zeros(nb + 1, np.int32)
+    col = np.zeros(nb, np.int32)
+    _call("op2_plan_export", _b(name), set_._p, len(args), arr, strategy, part_size,
+          perm.ctypes.data, thr.ctypes.data, off.ctypes.data, col.ctypes.data)
+    return {"perm": perm, "thrcol": thr, "blk_off": off, "blkcol": col}
+
+
+# user-kernel names for op_par_loop's first argument (the reference passes
+# the host function; only its identity matters there)
+save_soln, adt_calc, res_calc, bres_calc, update = (
+    "save_soln", "adt_calc", "res_calc", "bres_calc", "update")
+res = "res"

@@ -1,0 +1,200 @@
+"""GPU parity: the HIP runtime against the oracle and the reference fixtures.
+
+Every loop runs through the C ABI (op2_par_loop) on cuda:0.  FP64 results
+must match to 1e-12 relative (the runtime sums increments in another order
+than the reference, nothing else differs); the FP32 form is compared with
+the oracle's FP32 form and its error against FP64 is reported.
+"""
+
+import os
+
+import numpy as np
+import pytest
+
+from oplus2_matlab_amd import airfoil as AF
+from oplus2_matlab_amd import mesh as M
+from oplus2_matlab_amd import op2
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-12
+STRATS = [op2.STRATEGY_ATOMIC, op2.STRATEGY_COLOURED]
+
+
+def note(msg):
+    print(f"[gpu] {msg}", flush=True)
+
+
+@pytest.fixture
+def runtime():
+    op2.op_init([], 0)
+    yield
+    op2.op_exit()
+    op2.set_loop_config(None, 0, 0, 0)
+
+
+def configure(strategy, part=0):
+    for loop in ("res_calc", "bres_calc", "res"):
+        op2.set_loop_config(loop, strategy, part, 0)
+
+
+def relmax(a, b):
+    return float(np.max(np.abs(np.asarray(a) - np.asarray(b))) / max(np.max(np.abs(b)), 1e-300))
+
+
+# ---------------------------------------------------------------------------
+# single loops
+
+@pytest.mark.parametrize("strategy", STRATS)
+def test_single_loops_fp64(runtime, strategy):
+    note(f"single loops fp64 strategy={strategy} mesh 48x24 random")
+    configure(strategy, 128)
+    mesh = M.o_mesh(48, 24, "random", seed=9)
+    c = M.flow_constants().as_array()
+    rng = np.random.Generator(np.random.PCG64(3))
+    q0 = np.tile(c[6:10], (mesh.ncell, 1)) * (1 + 0.02 * rng.standard_normal((mesh.ncell, 4)))
+    res0 = 1e-3 * rng.standard_normal((mesh.ncell, 4))
+    b = AF.Airfoil(mesh, c, "double", q0=q0, res0=res0)
+
+    ox, oq = mesh.x.copy(), q0.copy()
+    oqold = np.zeros_like(oq)
+    O.save_soln("f64", oq, oqold)
+    b.save_soln()
+    assert np.array_equal(op2.op_fetch_data(b.p_qold).reshape(-1, 4), oqold)
+
+    oadt = np.zeros(mesh.ncell)
+    O.adt_calc("f64", mesh, ox, oq, oadt, c)
+    b.adt_calc()
+    adt = op2.op_fetch_data(b.p_adt)
+    assert relmax(adt, oadt) < 1e-15
+    note(f"adt_calc bitwise equal to the oracle: {np.array_equal(adt, oadt)}")
+
+    ores = res0.copy()
+    O.res_calc("f64", mesh, ox, oq, oadt, ores, c)
+    b.res_calc()
+    assert relmax(op2.op_fetch_data(b.p_res).reshape(-1, 4), ores) < 1e-13
+
+    O.bres_calc("f64", mesh, ox, oq, oadt, ores, c)
+    b.bres_calc()
+    res = op2.op_fetch_data(b.p_res).reshape(-1, 4)
+    assert relmax(res, ores) < 1e-13
+
+    rms_o = O.update("f64", oqold, oq, ores, oadt)
+    rms_g = b.update()
+    assert relmax(op2.op_fetch_data(b.p_q).reshape(-1, 4), oq) < 1e-12
+    assert abs(rms_g - rms_o) / rms_o < 1e-12
+    assert not op2.op_fetch_data(b.p_res).any()
+
+
+# ---------------------------------------------------------------------------
+# full runs against the reference fixtures
+
+@pytest.mark.parametrize("strategy", STRATS)
+@pytest.mark.parametrize("part", [64, 256])
+def test_airfoil_24x12_matches_reference(runtime, golden_dir, strategy, part):
+    note(f"airfoil fp64 24x12 random, 20 iterations, strategy={strategy} part={part}")
+    configure(strategy, part)
+    fx = np.load(os.path.join(golden_dir, "airfoil_24x12_random_s1.npz"))
+    mesh = M.o_mesh(24, 12, "random", seed=1)
+    hist, q, adt, res = AF.run_airfoil(mesh, fx["f64_consts"], int(fx["niter"]), "double")
+    assert relmax(hist, fx["f64_seq_rms"]) < TOL64
+    assert np.max(np.abs(hist - fx["f64_seq_rms"]) / fx["f64_seq_rms"]) < TOL64
+    assert relmax(q, fx["f64_seq_q"]) < TOL64
+    assert relmax(adt, fx["f64_seq_adt"]) < TOL64
+
+
+@pytest.mark.parametrize("strategy", STRATS)
+def test_airfoil_180k_1000_iterations_matches_reference(runtime, golden_dir, strategy):
+    note(f"airfoil fp64 600x300 random, 1000 iterations, strategy={strategy}")
+    configure(strategy, 0)
+    fx = np.load(os.path.join(golden_dir, "airfoil_600x300_random_s1_f64.npz"))
+    mesh = M.o_mesh(600, 300, "random", seed=1)
+    hist, q, adt, _ = AF.run_airfoil(mesh, fx["consts"], int(fx["niter"]), "double")
+    rel = np.abs(hist - fx["rms"]) / np.abs(fx["rms"])
+    note(f"rms history max rel err {rel.max():.3e}")
+    assert rel.max() < TOL64
+    s = fx["sample"]
+    assert relmax(q[s], fx["q_sample"]) < TOL64
+    assert relmax(adt[s], fx["adt_sample"]) < TOL64
+    assert np.allclose(q.sum(axis=0), fx["q_sum"], rtol=TOL64, atol=0)
+
+
+def test_airfoil_fp32_form(runtime):
+    """FP32 state and flux, FP64 update and rms: matches the oracle's FP32
+    form to float rounding and stays close to the FP64 oracle."""
+    note("airfoil fp32 form 48x24 random, 200 iterations")
+    mesh = M.o_mesh(48, 24, "random", seed=3)
+    c = M.flow_constants().as_array()
+    hist, q, _, _ = AF.run_airfoil(mesh, c, 200, "float")
+    _, hmix = O.airfoil_seq(mesh, c, 200, "mix")
+    st64, h64 = O.airfoil_seq(mesh, c, 200, "f64")
+    e_mix = np.max(np.abs(hist - hmix) / np.abs(hmix))
+    e64 = np.max(np.abs(hist - h64) / np.abs(h64))
+    dq = np.max(np.abs(q - st64.q))
+    note(f"fp32 form: rms rel err vs oracle fp32 form {e_mix:.2e}, vs fp64 {e64:.2e}, "
+         f"max|q-q64| {dq:.2e}")
+    assert e_mix < 1e-4
+    assert e64 < 1e-4
+    assert dq < 1e-4
+
+
+# ---------------------------------------------------------------------------
+# Jacobi
+
+@pytest.mark.parametrize("strategy", STRATS)
+def test_jacobi_matches_reference(runtime, golden_dir, strategy):
+    note(f"jacobi fp64 64x64 random, 200 iterations, strategy={strategy}")
+    configure(strategy, 1024)
+    fx = np.load(os.path.join(golden_dir, "jacobi_64_random_s1_f64.npz"))
+    prob = M.jacobi_grid(64, seed=1)
+    hs, hm, u, du = AF.run_jacobi(prob, int(fx["niter"]), 0.25, -0.25, "double")
+    assert relmax(hs, fx["seq_u_sum"]) < TOL64
+    assert np.array_equal(hm, fx["seq_u_max"]) or relmax(hm, fx["seq_u_max"]) < TOL64
+    assert relmax(u, fx["seq_u"]) < TOL64
+    assert not du.any()
+
+
+def test_jac1_known_answer(runtime):
+    """jac1 as shipped (NN=6, alpha = beta = 1): exact dyadic values."""
+    note("jac1 NN=6 known answer")
+    prob = M.jacobi_reference_problem(6)
+    hs, hm, u, _ = AF.run_jacobi(prob, 2, 1.0, 1.0, "double")
+    assert list(hm) == [0.5, 0.625]
+    assert abs(np.sqrt(hs[0] / 25) - 0.264575) < 1e-6 and abs(np.sqrt(hs[1] / 25) - 0.386491) < 1e-6
+    grid = u.reshape(5, 5)
+    assert grid[0, 0] == 0.625 and grid[2, 2] == 0.0 and grid[1, 1] == 0.125
+
+
+# ---------------------------------------------------------------------------
+# full size: properties and one-call parity at 2.88M cells
+
+@pytest.mark.parametrize("ordering", ["cartesian", "random"])
+def test_full_size_res_calc(runtime, ordering):
+    note(f"res_calc fp64 2400x1200 {ordering}: one call vs oracle, conservation")
+    mesh = M.o_mesh(2400, 1200, ordering, seed=1)
+    c = M.flow_constants().as_array()
+    rng = np.random.Generator(np.random.PCG64(5))
+    q0 = np.tile(c[6:10], (mesh.ncell, 1)) * (1 + 0.01 * rng.standard_normal((mesh.ncell, 4)))
+    b = AF.Airfoil(mesh, c, "double", q0=q0)
+    b.adt_calc()
+    b.res_calc()
+    res = op2.op_fetch_data(b.p_res).reshape(-1, 4)
+    adt = op2.op_fetch_data(b.p_adt)
+    oadt = np.zeros(mesh.ncell)
+    O.adt_calc("f64", mesh, mesh.x, q0, oadt, c)
+    ores = np.zeros_like(q0)
+    O.res_calc("f64", mesh, mesh.x, q0, oadt, ores, c)
+    assert relmax(adt, oadt) < 1e-15
+    assert relmax(res, ores) < 1e-12
+    assert np.abs(res.sum(axis=0)).max() < 1e-11 * np.abs(res).sum()
+
+
+# ---------------------------------------------------------------------------
+# errors
+
+def test_unknown_loop_is_an_error(runtime):
+    s = op2.op_decl_set(4, "s")
+    d = op2.op_decl_dat(s, 1, "double", np.zeros(4), "d")
+    with pytest.raises(op2.OP2Error, match="no device kernel"):
+        op2.op_par_loop(None, "my_loop", s, op2.op_arg_dat(d, -1, None, 1, "double", op2.OP_RW))

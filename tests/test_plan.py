@@ -1,0 +1,138 @@
+"""Execution plans of the runtime, built and verified on the host.
+
+The runtime's own verifier must report zero violations, and an independent
+numpy check here must agree: positions are a permutation of each block's
+elements, no two same-colour elements of a block share an OP_INC target, and
+no two same-colour blocks share one.
+"""
+
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from oplus2_matlab_amd import mesh as M
+from oplus2_matlab_amd import op2
+
+
+@pytest.fixture
+def host_runtime():
+    L = op2.lib()
+    argv = (C.c_char_p * 1)(b"t")
+    L.op_init_core(1, argv, 0)
+    yield
+    op2.op_exit()
+
+
+def declare(mesh):
+    nodes = op2.op_decl_set(mesh.nnode, "nodes")
+    edges = op2.op_decl_set(mesh.nedge, "edges")
+    bedges = op2.op_decl_set(mesh.nbedge, "bedges")
+    cells = op2.op_decl_set(mesh.ncell, "cells")
+    pedge = op2.op_decl_map(edges, nodes, 2, mesh.edge, "pedge")
+    pecell = op2.op_decl_map(edges, cells, 2, mesh.ecell, "pecell")
+    pbedge = op2.op_decl_map(bedges, nodes, 2, mesh.bedge, "pbedge")
+    pbecell = op2.op_decl_map(bedges, cells, 1, mesh.becell, "pbecell")
+    x = op2.op_decl_dat_host(nodes, 2, "double", mesh.x, "x")
+    q = op2.op_decl_dat_host(cells, 4, "double", np.zeros(4 * mesh.ncell), "q")
+    adt = op2.op_decl_dat_host(cells, 1, "double", np.zeros(mesh.ncell), "adt")
+    res = op2.op_decl_dat_host(cells, 4, "double", np.zeros(4 * mesh.ncell), "res")
+    bound = op2.op_decl_dat_host(bedges, 1, "int", mesh.bound, "bound")
+    R, I = op2.OP_READ, op2.OP_INC
+    res_args = [op2.op_arg_dat(x, 0, pedge, 2, "double", R),
+                op2.op_arg_dat(x, 1, pedge, 2, "double", R),
+                op2.op_arg_dat(q, 0, pecell, 4, "double", R),
+                op2.op_arg_dat(q, 1, pecell, 4, "double", R),
+                op2.op_arg_dat(adt, 0, pecell, 1, "double", R),
+                op2.op_arg_dat(adt, 1, pecell, 1, "double", R),
+                op2.op_arg_dat(res, 0, pecell, 4, "double", I),
+                op2.op_arg_dat(res, 1, pecell, 4, "double", I)]
+    bres_args = [op2.op_arg_dat(x, 0, pbedge, 2, "double", R),
+                 op2.op_arg_dat(x, 1, pbedge, 2, "double", R),
+                 op2.op_arg_dat(q, 0, pbecell, 4, "double", R),
+                 op2.op_arg_dat(adt, 0, pbecell, 1, "double", R),
+                 op2.op_arg_dat(res, 0, pbecell, 4, "double", I),
+                 op2.op_arg_dat(bound, -1, None, 1, "int", R)]
+    return dict(edges=edges, bedges=bedges, res_args=res_args, bres_args=bres_args)
+
+
+def check_plan(exp, targets, coloured):
+    """targets: (n, k) INC targets of each element."""
+    perm, thr, off, col = exp["perm"], exp["thrcol"], exp["blk_off"], exp["blkcol"]
+    n = perm.size
+    assert off[0] == 0 and off[-1] == n
+    for b in range(off.size - 1):
+        lo, hi = off[b], off[b + 1]
+        assert np.array_equal(np.sort(perm[lo:hi]), np.arange(lo, hi))
+    if not coloured:
+        return
+    blk_of = np.repeat(np.arange(off.size - 1), np.diff(off))
+    k = targets.shape[1]
+    # thread colours: (block, colour, target) must be owned by one element
+    e = perm
+    key = np.stack([np.repeat(blk_of, k), np.repeat(thr, k), targets[e].reshape(-1),
+                    np.repeat(e, k)], axis=1)
+    key = np.unique(key, axis=0)  # one element touching a target twice is fine
+    _, cnt = np.unique(key[:, :3], axis=0, return_counts=True)
+    assert cnt.max() == 1, "two elements of one thread colour share a target"
+    # block colours: (colour, target) owned by one block
+    bkey = np.unique(np.stack([np.repeat(col[blk_of], k), targets[e].reshape(-1),
+                               np.repeat(blk_of, k)], axis=1), axis=0)
+    _, cnt = np.unique(bkey[:, :2], axis=0, return_counts=True)
+    assert cnt.max() == 1, "two blocks of one colour share a target"
+
+
+@pytest.mark.parametrize("ordering", ["cartesian", "random", "rcm"])
+@pytest.mark.parametrize("strategy", [op2.STRATEGY_COLOURED, op2.STRATEGY_ATOMIC])
+@pytest.mark.parametrize("part", [64, 100, 256, 1000])
+def test_res_calc_plans(host_runtime, ordering, strategy, part):
+    mesh = M.o_mesh(60, 30, ordering, seed=2)
+    d = declare(mesh)
+    s = op2.plan_summary("res_calc", d["edges"], d["res_args"], strategy, part)
+    assert s["verify_errors"] == 0
+    assert s["nblocks"] == -(-mesh.nedge // part)
+    exp = op2.plan_export("res_calc", d["edges"], d["res_args"], strategy, part)
+    check_plan(exp, mesh.ecell, strategy == op2.STRATEGY_COLOURED)
+    if strategy == op2.STRATEGY_COLOURED:
+        # every cell has four edges: blocks that hold several of them need
+        # several thread colours, and neighbouring blocks need block colours
+        assert s["mean_thread_colours"] >= (2 if ordering != "random" else 1)
+        assert s["ncolors"] >= 2
+
+
+@pytest.mark.parametrize("strategy", [op2.STRATEGY_COLOURED, op2.STRATEGY_ATOMIC])
+def test_bres_calc_plan(host_runtime, strategy):
+    mesh = M.o_mesh(60, 1, "random", seed=4)  # one ring: cells carry two bedges
+    d = declare(mesh)
+    s = op2.plan_summary("bres_calc", d["bedges"], d["bres_args"], strategy, 32)
+    assert s["verify_errors"] == 0
+    exp = op2.plan_export("bres_calc", d["bedges"], d["bres_args"], strategy, 32)
+    check_plan(exp, mesh.becell.reshape(-1, 1), strategy == op2.STRATEGY_COLOURED)
+
+
+def test_single_element_and_small_sets(host_runtime):
+    a = op2.op_decl_set(1, "one")
+    b = op2.op_decl_set(1, "tgt")
+    m = op2.op_decl_map(a, b, 2, np.array([0, 0], np.int32), "m")
+    u = op2.op_decl_dat_host(b, 1, "double", np.zeros(1), "u")
+    du = op2.op_decl_dat_host(b, 1, "double", np.zeros(1), "du")
+    A = op2.op_decl_dat_host(a, 1, "double", np.zeros(1), "A")
+    beta = np.zeros(1)
+    args = [op2.op_arg_dat(A, -1, None, 1, "double", op2.OP_READ),
+            op2.op_arg_dat(u, 1, m, 1, "double", op2.OP_READ),
+            op2.op_arg_dat(du, 0, m, 1, "double", op2.OP_INC),
+            op2.op_arg_gbl(beta, 1, "double", op2.OP_READ)]
+    s = op2.plan_summary("res", a, args, op2.STRATEGY_COLOURED, 64)
+    assert s["nblocks"] == 1 and s["verify_errors"] == 0 and s["ncolors"] == 1
+
+
+def test_plan_is_cached(host_runtime):
+    mesh = M.o_mesh(12, 6, "random", seed=1)
+    d = declare(mesh)
+    op2.plan_summary("res_calc", d["edges"], d["res_args"], op2.STRATEGY_COLOURED, 64)
+    L = op2.lib()
+    n0 = C.c_int.in_dll(L, "OP_plan_index").value
+    op2.plan_summary("res_calc", d["edges"], d["res_args"], op2.STRATEGY_COLOURED, 64)
+    assert C.c_int.in_dll(L, "OP_plan_index").value == n0
+    op2.plan_summary("res_calc", d["edges"], d["res_args"], op2.STRATEGY_ATOMIC, 64)
+    assert C.c_int.in_dll(L, "OP_plan_index").value == n0 + 1
