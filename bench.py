@@ -3,14 +3,20 @@
 
 A step is one airfoil iteration (save_soln, then twice adt_calc, res_calc,
 bres_calc, update with the rms reduction; airfoil/airfoil.cpp:197-257) over
-the 2400 x 1200 O-mesh (2.88M cells, 5.76M edges), run through the OP2 C ABI
-with every dat resident in HBM.  value = cells x steps / seconds of the
-timed steps.
+the 2400 x 1200 O-mesh (2.88M cells, 5.76M edges), with every dat resident
+in HBM.  The iteration is captured once as a hipGraph with deferred
+reductions (the rms of every iteration lands in the runtime's device log)
+and replayed; value = cells x steps / seconds of the timed replays (FP64).
 
 roofline: res_calc, the hot loop.  achieved = algorithmic bytes of one call
 (each touched dat once per direction plus each map once: 76 B/edge FP64)
-divided by the mean device time of one call, from HIP events recorded on the
-runtime's stream around every call in the timed steps.
+divided by the mean device time of one call, from HIP events the runtime
+records on its stream around every loop call of an eager timed region.
+
+fp32_form: the same mesh with float state and flux (update and rms in
+double): its throughput, speed-up over FP64, and its error against the FP64
+run after --accuracy-iters iterations (rms history relative error and
+max |q - q64|).
 
 cpu_baseline: the reference's OpenMP schedule (the oracle's bit-exact
 restatement of airfoil/<loop>_kernel.cpp on the reference plan) on this
@@ -30,6 +36,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "airfoil cell-updates/s & res_calc HBM GB/s (FP64 vs FP32 form) on 1 MI355X"
 HBM_PEAK_GBS = 8000.0
+STRATEGIES = {"auto": 0, "coloured": 1, "atomic": 2, "direct": 3, "owner": 4, "owner-gather": 5}
 
 
 def parse():
@@ -39,12 +46,78 @@ def parse():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--mesh", default="2400x1200", help="NIxNJ O-mesh cells")
     ap.add_argument("--ordering", default="cartesian", choices=["cartesian", "random", "rcm"])
-    ap.add_argument("--dtype", default="double", choices=["double", "float"])
-    ap.add_argument("--strategy", default="atomic", choices=["atomic", "coloured"])
+    ap.add_argument("--strategy", default="auto", choices=list(STRATEGIES))
     ap.add_argument("--part", type=int, default=0, help="res_calc part size (0 = default)")
+    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-form measurement")
+    ap.add_argument("--accuracy-iters", type=int, default=1000,
+                    help="iterations of the FP32-vs-FP64 accuracy run (0 = skip)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="target CPU time of the cpu_baseline sample (0 = skip)")
     return ap.parse_args()
+
+
+def measure(mesh, consts, dtype, a):
+    """Eager timed region (per-loop HIP events) then graph timed region."""
+    from oplus2_matlab_amd import airfoil as AF
+    from oplus2_matlab_amd import op2
+
+    op2.op_init([], 0)
+    try:
+        if a.strategy != "auto" or a.part:
+            op2.set_loop_config("res_calc", STRATEGIES[a.strategy], a.part, 0)
+        af = AF.Airfoil(mesh, consts, dtype)
+        for _ in range(a.warmup):
+            af.iterate()
+        op2.sync()
+        op2.timing_reset()
+        t0 = time.perf_counter()
+        for _ in range(a.steps):
+            af.iterate()
+        op2.sync()
+        dt_eager = time.perf_counter() - t0
+        loops = {}
+        for name in ("save_soln", "adt_calc", "res_calc", "bres_calc", "update"):
+            st = op2.loop_stats(name)
+            if st:
+                n, secs, bytes_ = st
+                loops[name] = {"calls": n, "us_per_call": 1e6 * secs / n,
+                               "GBs": bytes_ * n / secs / 1e9, "bytes_per_call": bytes_}
+        gid = af.capture_iteration()
+        af.run_graph(gid, a.warmup)
+        op2.sync()
+        t0 = time.perf_counter()
+        op2.graph_launch(gid, a.steps)
+        op2.sync()
+        dt_graph = time.perf_counter() - t0
+        log = op2.reduction_log()
+        rms_last = float((log[-1] / mesh.ncell) ** 0.5) if log.size else None
+        op2.set_deferred_reductions(False)
+    finally:
+        op2.op_exit()
+    return {"value": mesh.ncell * a.steps / dt_graph, "ms_per_step": 1e3 * dt_graph / a.steps,
+            "eager_ms_per_step": 1e3 * dt_eager / a.steps, "loops": loops, "rms_last": rms_last}
+
+
+def accuracy(mesh, consts, iters):
+    """FP32 form against FP64 after `iters` iterations (graph replay)."""
+    import numpy as np
+
+    from oplus2_matlab_amd import airfoil as AF
+    from oplus2_matlab_amd import op2
+
+    out = {}
+    for dtype in ("double", "float"):
+        op2.op_init([], 0)
+        try:
+            out[dtype] = AF.run_airfoil(mesh, consts, iters, dtype, graph=True)
+        finally:
+            op2.op_exit()
+    h64, q64 = out["double"][0], out["double"][1]
+    h32, q32 = out["float"][0], out["float"][1]
+    return {"iterations": iters,
+            "rms_history_max_rel_err": float(np.max(np.abs(h32 - h64) / np.abs(h64))),
+            "rms_final_rel_err": float(abs(h32[-1] - h64[-1]) / abs(h64[-1])),
+            "max_abs_q_minus_q64": float(np.max(np.abs(q32.astype(np.float64) - q64)))}
 
 
 def cpu_baseline(mesh, consts, target_s):
@@ -64,7 +137,7 @@ def cpu_baseline(mesh, consts, target_s):
             "sample": f"{iters} iterations of the same FP64 airfoil mesh on the reference "
                       f"OpenMP schedule (oracle restatement, bit-exact to the reference, "
                       f"adaptive part size), {nthr} threads, {t:.1f} s",
-            "res_calc_gbs": None}
+            "res_calc_GBs": float(mesh.nedge * 76.0 * iters * 2 / secs[2] / 1e9)}
 
 
 def main():
@@ -75,55 +148,31 @@ def main():
               "the mesh is a design note, not part of this build", file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
-
-    from oplus2_matlab_amd import airfoil as AF
     from oplus2_matlab_amd import mesh as M
-    from oplus2_matlab_amd import op2
 
     ni, nj = (int(v) for v in a.mesh.lower().split("x"))
     mesh = M.o_mesh(ni, nj, a.ordering, seed=1)
     consts = M.flow_constants().as_array()
-    strategy = {"atomic": op2.STRATEGY_ATOMIC, "coloured": op2.STRATEGY_COLOURED}[a.strategy]
 
-    op2.op_init([], 0)
-    for loop in ("res_calc", "bres_calc"):
-        op2.set_loop_config(loop, strategy, a.part if loop == "res_calc" else 0, 0)
-    af = AF.Airfoil(mesh, consts, a.dtype)
-    for _ in range(a.warmup):
-        af.iterate()
-    op2.sync()
-    op2.timing_reset()
-    t0 = time.perf_counter()
-    hist = [af.iterate() for _ in range(a.steps)]
-    op2.sync()
-    dt = time.perf_counter() - t0
-
-    loops = {}
-    for name in ("save_soln", "adt_calc", "res_calc", "bres_calc", "update"):
-        st = op2.loop_stats(name)
-        if st:
-            n, secs, bytes_ = st
-            loops[name] = {"calls": n, "us_per_call": 1e6 * secs / n,
-                           "GBs": bytes_ * n / secs / 1e9, "bytes_per_call": bytes_}
-    rc = loops["res_calc"]
+    r64 = measure(mesh, consts, "double", a)
+    rc = r64["loops"]["res_calc"]
     achieved = rc["GBs"]
-    sreal = 8 if a.dtype == "double" else 4
     result = {
         "metric": METRIC,
-        "value": mesh.ncell * a.steps / dt,
+        "value": r64["value"],
         "unit": "cell-updates/s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": 1e3 * dt / a.steps,
+        "ms_per_step": r64["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if a.dtype == "double" else "f32 state/flux, f64 update+rms",
+        "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"airfoil {a.dtype} {ni}x{nj} O-mesh ({mesh.ncell} cells, "
-                               f"{mesh.nedge} edges), {a.ordering} ordering",
+        "config": {"workload": f"airfoil FP64 {ni}x{nj} O-mesh ({mesh.ncell} cells, "
+                               f"{mesh.nedge} edges), {a.ordering} ordering, iteration "
+                               f"replayed as a hipGraph",
                    "cells": mesh.ncell, "edges": mesh.nedge, "ordering": a.ordering,
                    "strategy": a.strategy, "part_size": a.part or None},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -131,15 +180,21 @@ def main():
                      "kernel": "res_calc", "bytes_per_launch": rc["bytes_per_call"],
                      "bytes_per_edge": rc["bytes_per_call"] / mesh.nedge,
                      "us_per_launch": rc["us_per_call"]},
-        "loops": loops,
-        "rms_last": hist[-1],
+        "eager_ms_per_step": r64["eager_ms_per_step"],
+        "loops": r64["loops"],
+        "rms_last": r64["rms_last"],
     }
-    op2.op_exit()
-    if a.cpu_seconds > 0:
-        result["cpu_baseline"] = cpu_baseline(mesh, consts, a.cpu_seconds)
-    else:
-        result["cpu_baseline"] = None
-    del np
+    if not a.no_fp32:
+        r32 = measure(mesh, consts, "float", a)
+        result["fp32_form"] = {"value": r32["value"], "unit": "cell-updates/s",
+                               "ms_per_step": r32["ms_per_step"],
+                               "speedup_vs_fp64": r32["value"] / r64["value"],
+                               "res_calc_GBs": r32["loops"]["res_calc"]["GBs"],
+                               "res_calc_us": r32["loops"]["res_calc"]["us_per_call"],
+                               "loops": r32["loops"]}
+        if a.accuracy_iters > 0:
+            result["fp32_form"]["accuracy"] = accuracy(mesh, consts, a.accuracy_iters)
+    result["cpu_baseline"] = cpu_baseline(mesh, consts, a.cpu_seconds) if a.cpu_seconds > 0 else None
     print(json.dumps(result))
 
 

@@ -136,6 +136,18 @@ void op2_clear_error(void);
 #define OP2_STRATEGY_COLOURED 1
 /* one launch; LDS atomics inside a block, global atomics across blocks */
 #define OP2_STRATEGY_ATOMIC 2
+/* one launch, no plan: every element gathers straight from global memory
+ * and increments with global atomics (loops that support it) */
+#define OP2_STRATEGY_DIRECT 3
+/* one launch, owner-compute: the OP_INC targets are cut into compact tiles,
+ * one per block; a block evaluates every element that increments one of its
+ * targets (elements between two tiles run in both) and adds each target's
+ * increments in a fixed order with a plain read-modify-write: no atomics,
+ * no colours, deterministic */
+#define OP2_STRATEGY_OWNER 4
+/* owner-compute tiles without LDS staging: elements gather their records
+ * straight from global memory (loops that support it; others use OWNER) */
+#define OP2_STRATEGY_OWNER_GATHER 5
 
 /* Per-loop tuning: strategy, part size (elements per block, 0 = default)
  * and threads per block (0 = default).  name NULL applies to every loop. */
@@ -154,11 +166,33 @@ void *op2_stream(void);
 /* Waits for all work on the runtime stream. */
 int op2_sync(void);
 
+/* Deferred reductions: while on, OP_INC / OP_MIN / OP_MAX globals are not
+ * written back to the caller's variable; each loop appends its results
+ * (one double per global argument, in argument order) to a device log, and
+ * loops never wait for the device.  Returns the previous setting. */
+int op2_set_deferred_reductions(int on);
+/* Waits for the stream, then copies up to max pending log entries (oldest
+ * first) to out; returns how many were copied (-1 on error, e.g. more than
+ * 65536 entries pending). */
+int op2_reduction_log(double *out, int max);
+
+/* Graph capture of a loop sequence on the runtime stream (hipGraph).
+ * op2_graph_begin starts capturing (deferred reductions required; every
+ * captured loop must have run once before, so that its plan and maps are
+ * already on the device), op2_graph_end instantiates the graph and returns
+ * its id, op2_graph_launch replays it count times.  OP_READ globals are
+ * captured by value.  op2_exit destroys every graph. */
+int op2_graph_begin(void);
+int op2_graph_end(void);
+int op2_graph_launch(int id, int count);
+int op2_graph_destroy(int id);
+
 /* Device pointer of a dat (for zero-copy interop). */
 void *op2_dat_device_ptr(op_dat dat);
 
-/* Plan introspection for the plan of the given strategy (COLOURED or
- * ATOMIC) and part size (0 = 512): block count, block colours, largest block, largest
+/* Plan introspection for the plan of the given strategy (COLOURED, ATOMIC
+ * or OWNER) and part size (0 = 512; targets per tile for OWNER): block
+ * count, block colours, largest block, largest
  * number of staged elements of one block (summed over its maps), mean
  * thread colours, and the number of violations found by the exact plan
  * verifier (0 = every block and thread colouring is valid and every local

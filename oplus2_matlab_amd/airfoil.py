@@ -121,17 +121,59 @@ class Airfoil:
             rms = self.update()
         return float(np.sqrt(rms / self.ncell))
 
+    def capture_iteration(self) -> int:
+        """Captures one iteration as a hipGraph (deferred reductions on; the
+        loops must have run once before).  Returns the graph id."""
+        op2.set_deferred_reductions(True)
+        op2.graph_begin()
+        try:
+            self.save_soln()
+            for _ in range(2):
+                self.adt_calc()
+                self.res_calc()
+                self.bres_calc()
+                self.update()
+        except Exception:
+            op2.graph_end()
+            raise
+        return op2.graph_end()
+
+    def run_graph(self, gid: int, niter: int, chunk: int = 8192) -> np.ndarray:
+        """Replays the captured iteration niter times; returns the rms
+        history from the reduction log (two updates per iteration; the
+        second stage's sum gives the iteration's rms)."""
+        op2.reduction_log()  # drop earlier entries
+        hist = []
+        done = 0
+        while done < niter:
+            n = min(chunk, niter - done)
+            op2.graph_launch(gid, n)
+            log = op2.reduction_log()
+            if log.size != 2 * n:
+                raise op2.OP2Error(f"reduction log holds {log.size} entries, expected {2 * n}")
+            hist.append(np.sqrt(log[1::2] / self.ncell))
+            done += n
+        return np.concatenate(hist)
+
     def fetch(self):
         for d in (self.p_q, self.p_qold, self.p_adt, self.p_res):
             op2.op_fetch_data(d)
         return self.q.reshape(-1, 4), self.qold.reshape(-1, 4), self.adt, self.res.reshape(-1, 4)
 
 
-def run_airfoil(mesh, consts, niter: int, dtype: str = "double"):
+def run_airfoil(mesh, consts, niter: int, dtype: str = "double", graph: bool = False):
     """Declares the mesh, runs `niter` iterations, returns (rms history,
-    q, adt, res) after fetching the dats.  The caller owns op_init/op_exit."""
+    q, adt, res) after fetching the dats.  With graph=True the first
+    iteration runs eagerly and the rest replay a captured hipGraph with
+    deferred reductions.  The caller owns op_init/op_exit."""
     a = Airfoil(mesh, consts, dtype)
-    hist = np.array([a.iterate() for _ in range(niter)])
+    if not graph or niter < 2:
+        hist = np.array([a.iterate() for _ in range(niter)])
+    else:
+        first = a.iterate()
+        gid = a.capture_iteration()
+        hist = np.concatenate([[first], a.run_graph(gid, niter - 1)])
+        op2.set_deferred_reductions(False)
     q, _, adt, res = a.fetch()
     return hist, q.copy(), adt.copy(), res.copy()
 

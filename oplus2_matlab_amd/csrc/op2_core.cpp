@@ -5,6 +5,7 @@
 
 #include <cstdarg>
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
 
 #include "op2_internal.h"
@@ -64,6 +65,9 @@ void ensure_device() {
 
 void *dev_alloc(size_t bytes) {
   ensure_device();
+  if (rt().capturing)
+    fail("op2: device allocation during graph capture (run every loop once before "
+         "op2_graph_begin so that its plan and maps are on the device)");
   void *p = nullptr;
   OP2_HIP_CHECK(hipMalloc(&p, bytes ? bytes : 256));
   return p;
@@ -75,12 +79,16 @@ void dev_free(void *p) {
 
 void h2d(void *dst, const void *src, size_t bytes) {
   if (!bytes) return;
+  if (rt().capturing)
+    fail("op2: host-to-device copy during graph capture (run every loop once before "
+         "op2_graph_begin so that its plan and maps are on the device)");
   OP2_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, rt().stream));
   OP2_HIP_CHECK(hipStreamSynchronize(rt().stream));
 }
 
 void d2h(void *dst, const void *src, size_t bytes) {
   if (!bytes) return;
+  if (rt().capturing) fail("op2: device-to-host copy during graph capture");
   OP2_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, rt().stream));
   OP2_HIP_CHECK(hipStreamSynchronize(rt().stream));
 }
@@ -95,6 +103,7 @@ void ensure_map_device(op_map m) {
 void *reduct_scratch(size_t bytes) {
   Runtime &r = rt();
   if (bytes > r.reduct_bytes) {
+    if (r.capturing) fail("op2: reduction scratch grown during graph capture");
     if (r.d_reduct) {
       OP2_HIP_CHECK(hipStreamSynchronize(r.stream));
       dev_free(r.d_reduct);
@@ -104,6 +113,17 @@ void *reduct_scratch(size_t bytes) {
     OP2_HIP_CHECK(hipMemsetAsync(r.d_reduct, 0, r.reduct_bytes, r.stream));
   }
   return r.d_reduct;
+}
+
+void ensure_reduction_log() {
+  Runtime &r = rt();
+  if (r.d_log) return;
+  if (r.capturing) fail("op2: reduction log created during graph capture");
+  r.d_log = (double *)dev_alloc(sizeof(double) * kLogCap);
+  r.d_log_count = (unsigned *)dev_alloc(256);
+  OP2_HIP_CHECK(hipMemsetAsync(r.d_log_count, 0, 256, r.stream));
+  OP2_HIP_CHECK(hipStreamSynchronize(r.stream));
+  r.log_read = 0;
 }
 
 const Const &get_const(const char *name) {
@@ -155,7 +175,7 @@ static hipEvent_t take_event() {
 void timing_begin(LoopTiming &t, hipEvent_t *start) {
   (void)t;
   *start = nullptr;
-  if (!rt().timing_enabled) return;
+  if (!rt().timing_enabled || rt().capturing) return;
   *start = take_event();
   OP2_HIP_CHECK(hipEventRecord(*start, rt().stream));
 }
@@ -171,7 +191,7 @@ void timing_end(LoopTiming &t, hipEvent_t start) {
 
 void timing_drain() {
   Runtime &r = rt();
-  if (!r.device_ready) return;
+  if (!r.device_ready || r.capturing) return;
   OP2_HIP_CHECK(hipStreamSynchronize(r.stream));
   for (auto &t : r.timings) {
     for (auto &p : t.pending) {
@@ -187,7 +207,17 @@ void timing_drain() {
 
 static void clear_all() {
   Runtime &r = rt();
+  if (r.capturing) {
+    hipGraph_t g = nullptr;
+    (void)hipStreamEndCapture(r.stream, &g);
+    if (g) (void)hipGraphDestroy(g);
+    r.capturing = false;
+  }
   if (r.device_ready) (void)hipStreamSynchronize(r.stream);
+  for (auto &g : r.graphs)
+    if (g) (void)hipGraphExecDestroy(g);
+  r.graphs.clear();
+  r.deferred = false;
   for (auto &p : r.plans) p->release();
   r.plans.clear();
   for (int i = 0; i < OP_dat_index; i++) {
@@ -521,7 +551,7 @@ void op2_clear_error(void) { rt().last_error.clear(); }
 
 int op2_set_loop_config(char const *name, int strategy, int part_size, int block_size) {
   API_BEGIN
-  if (strategy < 0 || strategy > OP2_STRATEGY_ATOMIC) fail("op2: unknown strategy %d", strategy);
+  if (strategy < 0 || strategy > OP2_STRATEGY_OWNER_GATHER) fail("op2: unknown strategy %d", strategy);
   if (part_size < 0 || part_size > 65535) fail("op2: part size %d out of range", part_size);
   if (block_size != 0 && (block_size < 64 || block_size > 1024 || block_size % 64))
     fail("op2: block size %d must be a multiple of 64 in [64, 1024]", block_size);
@@ -576,6 +606,97 @@ int op2_sync(void) {
 }
 
 void *op2_dat_device_ptr(op_dat dat) { return dat ? (void *)dat->data_d : nullptr; }
+
+int op2_set_deferred_reductions(int on) {
+  API_BEGIN
+  Runtime &r = rt();
+  if (r.capturing) fail("op2: reduction mode cannot change during graph capture");
+  int old = r.deferred ? 1 : 0;
+  if (on) {
+    ensure_device();
+    ensure_reduction_log();
+  }
+  r.deferred = on != 0;
+  return old;
+  API_END(-1)
+}
+
+int op2_reduction_log(double *out, int max) {
+  API_BEGIN
+  Runtime &r = rt();
+  if (r.capturing) fail("op2: reduction log read during graph capture");
+  if (!r.d_log) return 0;
+  unsigned cnt32 = 0;
+  OP2_HIP_CHECK(hipStreamSynchronize(r.stream));
+  OP2_HIP_CHECK(hipMemcpy(&cnt32, r.d_log_count, sizeof cnt32, hipMemcpyDeviceToHost));
+  // the device counter is 32-bit; the host cursor carries the high bits
+  unsigned long long cnt = (r.log_read & ~0xffffffffull) | cnt32;
+  if (cnt < r.log_read) cnt += 1ull << 32;
+  unsigned long long pending = cnt - r.log_read;
+  if (pending > kLogCap)
+    fail("op2: reduction log overflow (%llu results pending, capacity %u); read it more often",
+         pending, kLogCap);
+  int n = (int)std::min<unsigned long long>(pending, (unsigned long long)(max < 0 ? 0 : max));
+  std::vector<double> all(kLogCap);
+  if (n > 0) {
+    OP2_HIP_CHECK(hipMemcpy(all.data(), r.d_log, sizeof(double) * kLogCap, hipMemcpyDeviceToHost));
+    for (int i = 0; i < n; i++) out[i] = all[(r.log_read + i) & (kLogCap - 1)];
+  }
+  r.log_read += n;
+  return n;
+  API_END(-1)
+}
+
+int op2_graph_begin(void) {
+  API_BEGIN
+  Runtime &r = rt();
+  ensure_device();
+  if (r.capturing) fail("op2: graph capture already active");
+  if (!r.deferred)
+    fail("op2: graph capture needs deferred reductions (op2_set_deferred_reductions(1))");
+  timing_drain();
+  reduct_scratch(256 + sizeof(double) * 2048 * 4);
+  OP2_HIP_CHECK(hipStreamSynchronize(r.stream));
+  OP2_HIP_CHECK(hipStreamBeginCapture(r.stream, hipStreamCaptureModeThreadLocal));
+  r.capturing = true;
+  return 0;
+  API_END(-1)
+}
+
+int op2_graph_end(void) {
+  API_BEGIN
+  Runtime &r = rt();
+  if (!r.capturing) fail("op2: no graph capture active");
+  hipGraph_t g = nullptr;
+  r.capturing = false;
+  OP2_HIP_CHECK(hipStreamEndCapture(r.stream, &g));
+  hipGraphExec_t ex = nullptr;
+  OP2_HIP_CHECK(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
+  OP2_HIP_CHECK(hipGraphDestroy(g));
+  r.graphs.push_back(ex);
+  return (int)r.graphs.size() - 1;
+  API_END(-1)
+}
+
+int op2_graph_launch(int id, int count) {
+  API_BEGIN
+  Runtime &r = rt();
+  if (id < 0 || id >= (int)r.graphs.size() || !r.graphs[id]) fail("op2: no graph %d", id);
+  for (int i = 0; i < count; i++) OP2_HIP_CHECK(hipGraphLaunch(r.graphs[id], r.stream));
+  return 0;
+  API_END(-1)
+}
+
+int op2_graph_destroy(int id) {
+  API_BEGIN
+  Runtime &r = rt();
+  if (id < 0 || id >= (int)r.graphs.size() || !r.graphs[id]) fail("op2: no graph %d", id);
+  OP2_HIP_CHECK(hipStreamSynchronize(r.stream));
+  OP2_HIP_CHECK(hipGraphExecDestroy(r.graphs[id]));
+  r.graphs[id] = nullptr;
+  return 0;
+  API_END(-1)
+}
 
 int op2_par_loop(char const *name, op_set set, int nargs, op_arg *args) {
   API_BEGIN

@@ -156,6 +156,245 @@ __global__ __launch_bounds__(NT) void k_res_calc(
   }
 }
 
+// Owner-compute edge loop (res_calc).  One workgroup per tile of cells:
+//  1. stage x of the tile's nodes and q, adt of its cells (owned + halo);
+//  2. one flux per element position into LDS (elements between two tiles are
+//     evaluated by both);
+//  3. one thread per owned cell sums its increments in the plan's fixed
+//     order and adds them to res with a plain read-modify-write.
+// No atomics, no colours, one launch, deterministic.
+template <typename T, int OWN>
+__global__ __launch_bounds__(NT) void k_res_calc_owner(
+    const int *__restrict__ blk_off, int nblk, const int *__restrict__ node_off,
+    const int *__restrict__ node_list, const int *__restrict__ cell_off,
+    const int *__restrict__ cell_list, const ushort4 *__restrict__ loc,
+    const int *__restrict__ owned_off, const int *__restrict__ owned,
+    const int *__restrict__ csr_off, const uint16_t *__restrict__ csr, const T *__restrict__ x,
+    const T *__restrict__ q, const T *__restrict__ adt, T *__restrict__ res,
+    AirfoilConsts<T> k) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  constexpr int pad = 16 / sizeof(T);
+  const int b = xcd_block(blockIdx.x, nblk);
+  const int p0 = blk_off[b], np = blk_off[b + 1] - p0;
+  const int n0 = node_off[b], nn = node_off[b + 1] - n0;
+  const int c0 = cell_off[b], nc = cell_off[b + 1] - c0;
+  const int o0 = owned_off[b], no = owned_off[b + 1] - o0;
+  T *xs = reinterpret_cast<T *>(smem);
+  T *qs = xs + 2 * ((nn + 1) & ~1);
+  T *as = qs + 4 * nc;
+  T *fs = as + ((nc + pad - 1) / pad) * pad;
+  const int tid = threadIdx.x;
+  // the res of this thread's owned cells is read now, so that its latency
+  // overlaps the staging and the flux phases (OWN owned cells per thread)
+  T rpre[OWN][4];
+  int gown[OWN];
+#pragma unroll
+  for (int j = 0; j < OWN; j++) {
+    const int i = tid + j * NT;
+    gown[j] = i < no ? owned[o0 + i] : -1;
+    if (gown[j] >= 0) Vec<T>::load4(res + 4 * (size_t)gown[j], rpre[j]);
+  }
+  for (int i = tid; i < nn; i += NT) {
+    T v[2];
+    Vec<T>::load2(x + 2 * (size_t)node_list[n0 + i], v);
+    Vec<T>::store2(xs + 2 * i, v);
+  }
+  for (int i = tid; i < nc; i += NT) {
+    const int g = cell_list[c0 + i];
+    T v[4];
+    Vec<T>::load4(q + 4 * (size_t)g, v);
+    Vec<T>::store4(qs + 4 * i, v);
+    as[i] = adt[g];
+  }
+  __syncthreads();
+  for (int p = tid; p < np; p += NT) {
+    const ushort4 L = loc[p0 + p];
+    T x1[2], x2[2], q1[4], q2[4], f[4];
+    Vec<T>::load2(xs + 2 * L.x, x1);
+    Vec<T>::load2(xs + 2 * L.y, x2);
+    Vec<T>::load4(qs + 4 * L.z, q1);
+    Vec<T>::load4(qs + 4 * L.w, q2);
+    edge_flux(x1, x2, q1, q2, as[L.z], as[L.w], k, f);
+    Vec<T>::store4(fs + 4 * p, f);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < OWN; j++) {
+    const int i = tid + j * NT;
+    if (gown[j] < 0) continue;
+    T acc[4] = {T(0), T(0), T(0), T(0)};
+    const int j1 = csr_off[o0 + i + 1];
+    for (int c = csr_off[o0 + i]; c < j1; c++) {
+      const unsigned e = csr[c];
+      T f[4];
+      Vec<T>::load4(fs + 4 * (e >> 2), f);
+      if (e & 3u) {
+#pragma unroll
+        for (int d = 0; d < 4; d++) acc[d] -= f[d];
+      } else {
+#pragma unroll
+        for (int d = 0; d < 4; d++) acc[d] += f[d];
+      }
+    }
+#pragma unroll
+    for (int d = 0; d < 4; d++) rpre[j][d] += acc[d];
+    Vec<T>::store4(res + 4 * (size_t)gown[j], rpre[j]);
+  }
+}
+
+// Owner-compute edge loop without staging (res_calc): the same tiles, but
+// each element position carries the global ids of its nodes and cells, and
+// x, q, adt are gathered straight from global memory (the tile's records
+// stay in the L1/L2); fluxes still meet in LDS for the fixed-order sums.
+template <typename T, int OWN>
+__global__ __launch_bounds__(NT) void k_res_calc_owner_direct(
+    const int *__restrict__ blk_off, int nblk, const int4 *__restrict__ ids,
+    const int *__restrict__ owned_off, const int *__restrict__ owned,
+    const int *__restrict__ csr_off, const uint16_t *__restrict__ csr, const T *__restrict__ x,
+    const T *__restrict__ q, const T *__restrict__ adt, T *__restrict__ res,
+    AirfoilConsts<T> k) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int b = xcd_block(blockIdx.x, nblk);
+  const int p0 = blk_off[b], np = blk_off[b + 1] - p0;
+  const int o0 = owned_off[b], no = owned_off[b + 1] - o0;
+  T *fs = reinterpret_cast<T *>(smem);
+  const int tid = threadIdx.x;
+  T rpre[OWN][4];
+  int gown[OWN];
+#pragma unroll
+  for (int j = 0; j < OWN; j++) {
+    const int i = tid + j * NT;
+    gown[j] = i < no ? owned[o0 + i] : -1;
+    if (gown[j] >= 0) Vec<T>::load4(res + 4 * (size_t)gown[j], rpre[j]);
+  }
+  for (int p = tid; p < np; p += NT) {
+    const int4 g = ids[p0 + p];
+    T x1[2], x2[2], q1[4], q2[4], f[4];
+    Vec<T>::load2(x + 2 * (size_t)g.x, x1);
+    Vec<T>::load2(x + 2 * (size_t)g.y, x2);
+    Vec<T>::load4(q + 4 * (size_t)g.z, q1);
+    Vec<T>::load4(q + 4 * (size_t)g.w, q2);
+    edge_flux(x1, x2, q1, q2, adt[g.z], adt[g.w], k, f);
+    Vec<T>::store4(fs + 4 * p, f);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < OWN; j++) {
+    const int i = tid + j * NT;
+    if (gown[j] < 0) continue;
+    T acc[4] = {T(0), T(0), T(0), T(0)};
+    const int j1 = csr_off[o0 + i + 1];
+    for (int c = csr_off[o0 + i]; c < j1; c++) {
+      const unsigned e = csr[c];
+      T f[4];
+      Vec<T>::load4(fs + 4 * (e >> 2), f);
+      if (e & 3u) {
+#pragma unroll
+        for (int d = 0; d < 4; d++) acc[d] -= f[d];
+      } else {
+#pragma unroll
+        for (int d = 0; d < 4; d++) acc[d] += f[d];
+      }
+    }
+#pragma unroll
+    for (int d = 0; d < 4; d++) rpre[j][d] += acc[d];
+    Vec<T>::store4(res + 4 * (size_t)gown[j], rpre[j]);
+  }
+}
+
+// Owner-compute boundary loop (bres_calc): one increment per element.
+template <typename T>
+__global__ __launch_bounds__(NT) void k_bres_calc_owner(
+    const int *__restrict__ blk_off, int nblk, const int *__restrict__ perm,
+    const int *__restrict__ node_off, const int *__restrict__ node_list,
+    const int *__restrict__ cell_off, const int *__restrict__ cell_list,
+    const ushort4 *__restrict__ loc, const int *__restrict__ owned_off,
+    const int *__restrict__ owned, const int *__restrict__ csr_off,
+    const uint16_t *__restrict__ csr, const T *__restrict__ x, const T *__restrict__ q,
+    const T *__restrict__ adt, T *__restrict__ res, const int *__restrict__ bound,
+    AirfoilConsts<T> k) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  constexpr int pad = 16 / sizeof(T);
+  const int b = xcd_block(blockIdx.x, nblk);
+  const int p0 = blk_off[b], np = blk_off[b + 1] - p0;
+  const int n0 = node_off[b], nn = node_off[b + 1] - n0;
+  const int c0 = cell_off[b], nc = cell_off[b + 1] - c0;
+  const int o0 = owned_off[b], no = owned_off[b + 1] - o0;
+  T *xs = reinterpret_cast<T *>(smem);
+  T *qs = xs + 2 * ((nn + 1) & ~1);
+  T *as = qs + 4 * nc;
+  T *fs = as + ((nc + pad - 1) / pad) * pad;
+  const int tid = threadIdx.x;
+  for (int i = tid; i < nn; i += NT) {
+    T v[2];
+    Vec<T>::load2(x + 2 * (size_t)node_list[n0 + i], v);
+    Vec<T>::store2(xs + 2 * i, v);
+  }
+  for (int i = tid; i < nc; i += NT) {
+    const int g = cell_list[c0 + i];
+    T v[4];
+    Vec<T>::load4(q + 4 * (size_t)g, v);
+    Vec<T>::store4(qs + 4 * i, v);
+    as[i] = adt[g];
+  }
+  __syncthreads();
+  for (int p = tid; p < np; p += NT) {
+    const ushort4 L = loc[p0 + p];
+    T x1[2], x2[2], q1[4], f[4];
+    Vec<T>::load2(xs + 2 * L.x, x1);
+    Vec<T>::load2(xs + 2 * L.y, x2);
+    Vec<T>::load4(qs + 4 * L.z, q1);
+    boundary_flux(x1, x2, q1, as[L.z], bound[perm[p0 + p]], k, f);
+    Vec<T>::store4(fs + 4 * p, f);
+  }
+  __syncthreads();
+  for (int i = tid; i < no; i += NT) {
+    T acc[4] = {T(0), T(0), T(0), T(0)};
+    const int j1 = csr_off[o0 + i + 1];
+    for (int j = csr_off[o0 + i]; j < j1; j++) {
+      T f[4];
+      Vec<T>::load4(fs + 4 * (csr[j] >> 2), f);
+#pragma unroll
+      for (int d = 0; d < 4; d++) acc[d] += f[d];
+    }
+    const size_t g = 4 * (size_t)owned[o0 + i];
+    T r[4];
+    Vec<T>::load4(res + g, r);
+#pragma unroll
+    for (int d = 0; d < 4; d++) r[d] += acc[d];
+    Vec<T>::store4(res + g, r);
+  }
+}
+
+// Direct edge loop (res_calc, no plan): each thread reads its edge's maps,
+// gathers x, q and adt straight from global memory and adds its flux to
+// both cells with global atomics.  No barriers and no LDS, so every wave
+// keeps its loads in flight; the L2 supplies the reuse between edges.
+template <typename T>
+__global__ __launch_bounds__(NT) void k_res_calc_direct(const int2 *__restrict__ pedge,
+                                                        const int2 *__restrict__ pecell,
+                                                        const T *__restrict__ x,
+                                                        const T *__restrict__ q,
+                                                        const T *__restrict__ adt,
+                                                        T *__restrict__ res, int nedge,
+                                                        AirfoilConsts<T> k) {
+  const int e = xcd_block(blockIdx.x, gridDim.x) * NT + threadIdx.x;
+  if (e >= nedge) return;
+  const int2 n = pedge[e];
+  const int2 c = pecell[e];
+  T x1[2], x2[2], q1[4], q2[4], f[4];
+  Vec<T>::load2(x + 2 * (size_t)n.x, x1);
+  Vec<T>::load2(x + 2 * (size_t)n.y, x2);
+  Vec<T>::load4(q + 4 * (size_t)c.x, q1);
+  Vec<T>::load4(q + 4 * (size_t)c.y, q2);
+  const T a1 = adt[c.x], a2 = adt[c.y];
+  edge_flux(x1, x2, q1, q2, a1, a2, k, f);
+#pragma unroll
+  for (int d = 0; d < 4; d++) atomicAdd(res + 4 * (size_t)c.x + d, f[d]);
+#pragma unroll
+  for (int d = 0; d < 4; d++) atomicAdd(res + 4 * (size_t)c.y + d, -f[d]);
+}
+
 // Staged boundary-edge loop (bres_calc): as k_res_calc with one increment per
 // element and the direct `bound` read through the plan's element order.
 template <typename T, bool COLOURED>
@@ -232,14 +471,13 @@ __global__ __launch_bounds__(NT) void k_bres_calc(
 }
 
 // update: q = qold - res/adt, res = 0, rms += sum of squared moves; the
-// move and the new q are formed in double and q rounded once to T.
+// move and the new q are formed in double and q rounded once to T.  Each
+// workgroup leaves its partial sum in partials[blockIdx.x]; k_fold adds them.
 template <typename T>
 __global__ __launch_bounds__(NT) void k_update(const T *__restrict__ qold, T *__restrict__ q,
                                                T *__restrict__ res, const T *__restrict__ adt,
-                                               int ncell, double *partials, unsigned *ticket,
-                                               double *out) {
+                                               int ncell, double *__restrict__ partials) {
   __shared__ double lds[NT / kWave];
-  __shared__ bool last;
   double acc = 0.0;
   for (int c = blockIdx.x * NT + threadIdx.x; c < ncell; c += gridDim.x * NT) {
     T qo[4], r[4], qn[4];
@@ -256,7 +494,39 @@ __global__ __launch_bounds__(NT) void k_update(const T *__restrict__ qold, T *__
     Vec<T>::store4(q + 4 * (size_t)c, qn);
     Vec<T>::store4(res + 4 * (size_t)c, r);
   }
-  grid_reduce<kRedSum, NT>(acc, partials, ticket, out, lds, &last);
+  acc = block_reduce<kRedSum, NT>(acc, lds);
+  if (threadIdx.x == 0) partials[blockIdx.x] = acc;
+}
+
+// Folds the per-workgroup partials of NSLOT reductions in workgroup order
+// (deterministic for a given grid): slot s sums, or takes the maximum if bit
+// s of maxmask is set, partials[s*nparts .. s*nparts+nparts) into out[s].
+// One workgroup; the kernel boundary after the producer makes the partials
+// visible without device-scope fences.
+template <int NSLOT>
+__global__ __launch_bounds__(NT) void k_fold(const double *__restrict__ partials, int nparts,
+                                             unsigned maxmask, double *__restrict__ out,
+                                             double *__restrict__ log,
+                                             unsigned *__restrict__ log_count) {
+  __shared__ double lds[NT / kWave];
+#pragma unroll
+  for (int s = 0; s < NSLOT; s++) {
+    const bool mx = (maxmask >> s) & 1u;
+    double acc = mx ? red_identity<kRedMax>() : 0.0;
+    for (int i = threadIdx.x; i < nparts; i += NT) {
+      const double v = partials[s * nparts + i];
+      acc = mx ? red_apply<kRedMax>(acc, v) : acc + v;
+    }
+    acc = mx ? block_reduce<kRedMax, NT>(acc, lds) : block_reduce<kRedSum, NT>(acc, lds);
+    if (threadIdx.x == 0) {
+      out[s] = acc;
+      if (log) {
+        const unsigned i = atomicAdd(log_count, 1u);
+        log[i & (kLogCap - 1)] = acc;
+      }
+    }
+    __syncthreads();
+  }
 }
 
 // Staged Jacobi edge loop (jac1 res): du[pp0] += beta * A * u[pp1]; one map,
@@ -306,14 +576,43 @@ __global__ __launch_bounds__(NT) void k_jac_res(
   }
 }
 
+// Owner-compute Jacobi edge loop (jac1 res): nodes cut into runs, each
+// workgroup sums the contributions to its nodes in a fixed order.
+template <typename T>
+__global__ __launch_bounds__(NT) void k_jac_res_owner(
+    const int *__restrict__ blk_off, int nblk, const int *__restrict__ perm,
+    const int *__restrict__ node_off, const int *__restrict__ node_list,
+    const ushort2 *__restrict__ loc, const int *__restrict__ owned_off,
+    const int *__restrict__ owned, const int *__restrict__ csr_off,
+    const uint16_t *__restrict__ csr, const T *__restrict__ A, const T *__restrict__ u,
+    T *__restrict__ du, T beta) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int b = xcd_block(blockIdx.x, nblk);
+  const int p0 = blk_off[b], np = blk_off[b + 1] - p0;
+  const int n0 = node_off[b], nn = node_off[b + 1] - n0;
+  const int o0 = owned_off[b], no = owned_off[b + 1] - o0;
+  T *us = reinterpret_cast<T *>(smem);
+  T *fs = us + nn;
+  const int tid = threadIdx.x;
+  for (int i = tid; i < nn; i += NT) us[i] = u[node_list[n0 + i]];
+  __syncthreads();
+  for (int p = tid; p < np; p += NT) fs[p] = beta * A[perm[p0 + p]] * us[loc[p0 + p].x];
+  __syncthreads();
+  for (int i = tid; i < no; i += NT) {
+    T acc = T(0);
+    const int j1 = csr_off[o0 + i + 1];
+    for (int j = csr_off[o0 + i]; j < j1; j++) acc += fs[csr[j] >> 2];
+    du[owned[o0 + i]] += acc;
+  }
+}
+
 // jac1 update: u += du + alpha r, du = 0, u_sum += u^2, u_max = max(u_max, u)
+// (partials: u_sum of each workgroup, then u_max of each workgroup)
 template <typename T>
 __global__ __launch_bounds__(NT) void k_jac_update(const T *__restrict__ r, T *__restrict__ du,
                                                    T *__restrict__ u, int n, T alpha,
-                                                   double *partials, unsigned *tickets,
-                                                   double *out) {
+                                                   double *__restrict__ partials) {
   __shared__ double lds[NT / kWave];
-  __shared__ bool last;
   double s = 0.0, m = red_identity<kRedMax>();
   for (int i = blockIdx.x * NT + threadIdx.x; i < n; i += gridDim.x * NT) {
     T ui = u[i];
@@ -323,9 +622,13 @@ __global__ __launch_bounds__(NT) void k_jac_update(const T *__restrict__ r, T *_
     s += (double)ui * (double)ui;
     m = (m > (double)ui) ? m : (double)ui;
   }
-  grid_reduce<kRedSum, NT>(s, partials, tickets, out, lds, &last);
+  s = block_reduce<kRedSum, NT>(s, lds);
   __syncthreads();
-  grid_reduce<kRedMax, NT>(m, partials + gridDim.x, tickets + 1, out + 1, lds, &last);
+  m = block_reduce<kRedMax, NT>(m, lds);
+  if (threadIdx.x == 0) {
+    partials[blockIdx.x] = s;
+    partials[gridDim.x + blockIdx.x] = m;
+  }
 }
 
 // ============================================================================
@@ -420,9 +723,13 @@ void check_launch(const char *loop) {
   if (e != hipSuccess) fail("op2: kernel launch for loop %s failed: %s", loop, hipGetErrorString(e));
 }
 
-int strategy_of(const std::string &loop) {
+// Default strategy when none is configured: FP64 increments go through
+// LDS and global atomics (one launch); FP32 through block and thread colours
+// (float global atomics are slow on this part).
+int strategy_of(const std::string &loop, int real_bytes) {
   int s = config_for(loop).strategy;
-  return s ? s : OP2_STRATEGY_ATOMIC;
+  if (s) return s;
+  return real_bytes == 8 ? OP2_STRATEGY_ATOMIC : OP2_STRATEGY_COLOURED;
 }
 
 int part_of(const std::string &loop, int dflt) {
@@ -513,9 +820,69 @@ void launch_res_calc(Plan *p, op_arg *a, bool coloured) {
   }
 }
 
+// LDS bytes of an owner-compute block: staged groups, padded, plus one
+// increment record per element position
+template <typename T>
+size_t owner_lds(const Plan *p, int node_words, int cell_words, int inc_words) {
+  constexpr int pad = 16 / sizeof(T);
+  size_t w = 0;
+  if (node_words) w += (size_t)node_words * ((p->groups[p->slots[0].first].max_count + 1) & ~1);
+  if (cell_words) {
+    const int mc = p->groups[p->owner_group].max_count;
+    w += (size_t)cell_words * mc + ((mc + pad - 1) / pad) * pad;
+  }
+  w += (size_t)inc_words * p->max_positions;
+  return w * sizeof(T);
+}
+
+template <typename K>
+void allow_lds(K kern, size_t lds) {
+  if (lds > 160 * 1024) fail("op2: block needs %zu bytes of LDS; reduce the part size", lds);
+  if (lds > 64 * 1024)
+    OP2_HIP_CHECK(hipFuncSetAttribute((const void *)kern,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+}
+
 template <typename T>
 void run_res_calc(const char *name, op_set set, int nargs, op_arg *a) {
-  const int strategy = strategy_of(name);
+  const int strategy = strategy_of(name, sizeof(T));
+  if (strategy == OP2_STRATEGY_OWNER) {
+    Plan *p = plan_for(name, set, nargs, a, strategy, part_of(name, 256));
+    const PlanGroup &nodes = p->groups[p->slots[0].first];
+    const PlanGroup &cells = p->groups[p->owner_group];
+    if (p->max_owned > 2 * NT) fail("op2: owner tile of %s exceeds %d targets", name, 2 * NT);
+    const size_t lds = owner_lds<T>(p, 2, 4, 4);
+    auto kern = p->max_owned > NT ? k_res_calc_owner<T, 2> : k_res_calc_owner<T, 1>;
+    allow_lds(kern, lds);
+    hipLaunchKernelGGL(kern, dim3(p->nblocks), dim3(NT), lds, rt().stream,
+                       p->d_blk_off, p->nblocks, nodes.d_off, nodes.d_list, cells.d_off,
+                       cells.d_list, (const ushort4 *)p->d_loc, p->d_owned_off, p->d_owned,
+                       p->d_csr_off, p->d_csr, (const T *)a[0].dat->data_d,
+                       (const T *)a[2].dat->data_d, (const T *)a[4].dat->data_d,
+                       (T *)a[6].dat->data_d, airfoil_consts<T>());
+    return;
+  }
+  if (strategy == OP2_STRATEGY_OWNER_GATHER) {
+    Plan *p = plan_for(name, set, nargs, a, strategy, part_of(name, 256));
+    if (p->max_owned > 2 * NT) fail("op2: owner tile of %s exceeds %d targets", name, 2 * NT);
+    const size_t lds = sizeof(T) * 4 * (size_t)p->max_positions;
+    auto kern = p->max_owned > NT ? k_res_calc_owner_direct<T, 2> : k_res_calc_owner_direct<T, 1>;
+    allow_lds(kern, lds);
+    hipLaunchKernelGGL(kern, dim3(p->nblocks), dim3(NT), lds, rt().stream, p->d_blk_off,
+                       p->nblocks, (const int4 *)p->d_ids, p->d_owned_off, p->d_owned,
+                       p->d_csr_off, p->d_csr, (const T *)a[0].dat->data_d,
+                       (const T *)a[2].dat->data_d, (const T *)a[4].dat->data_d,
+                       (T *)a[6].dat->data_d, airfoil_consts<T>());
+    return;
+  }
+  if (strategy == OP2_STRATEGY_DIRECT) {
+    hipLaunchKernelGGL(k_res_calc_direct<T>, dim3(grid_for(set->size, 1 << 30)), dim3(NT), 0,
+                       rt().stream, (const int2 *)a[0].map->map_d, (const int2 *)a[2].map->map_d,
+                       (const T *)a[0].dat->data_d, (const T *)a[2].dat->data_d,
+                       (const T *)a[4].dat->data_d, (T *)a[6].dat->data_d, set->size,
+                       airfoil_consts<T>());
+    return;
+  }
   const int part = part_of(name, sizeof(T) == 8 ? 256 : 512);
   if (part > 4 * NT) fail("op2: part size %d of %s exceeds %d", part, name, 4 * NT);
   Plan *p = plan_for(name, set, nargs, a, strategy, part);
@@ -528,9 +895,32 @@ void run_res_calc(const char *name, op_set set, int nargs, op_arg *a) {
     launch_res_calc<T, 4>(p, a, col);
 }
 
+// loops with fewer kernel forms than res_calc map the other strategies on
+// their nearest form
+int narrow_strategy(int s) {
+  if (s == OP2_STRATEGY_OWNER_GATHER) return OP2_STRATEGY_OWNER;
+  if (s == OP2_STRATEGY_DIRECT) return OP2_STRATEGY_ATOMIC;
+  return s;
+}
+
 template <typename T>
 void run_bres_calc(const char *name, op_set set, int nargs, op_arg *a) {
-  const int strategy = strategy_of(name);
+  const int strategy = narrow_strategy(strategy_of(name, sizeof(T)));
+  if (strategy == OP2_STRATEGY_OWNER) {
+    Plan *p = plan_for(name, set, nargs, a, strategy, part_of(name, NT));
+    const PlanGroup &nodes = p->groups[p->slots[0].first];
+    const PlanGroup &cells = p->groups[p->owner_group];
+    const size_t lds = owner_lds<T>(p, 2, 4, 4);
+    allow_lds(k_bres_calc_owner<T>, lds);
+    hipLaunchKernelGGL(k_bres_calc_owner<T>, dim3(p->nblocks), dim3(NT), lds, rt().stream,
+                       p->d_blk_off, p->nblocks, p->d_perm, nodes.d_off, nodes.d_list,
+                       cells.d_off, cells.d_list, (const ushort4 *)p->d_loc, p->d_owned_off,
+                       p->d_owned, p->d_csr_off, p->d_csr, (const T *)a[0].dat->data_d,
+                       (const T *)a[2].dat->data_d, (const T *)a[3].dat->data_d,
+                       (T *)a[4].dat->data_d, (const int *)a[5].dat->data_d,
+                       airfoil_consts<T>());
+    return;
+  }
   Plan *p = plan_for(name, set, nargs, a, strategy, part_of(name, NT));
   const bool col = strategy == OP2_STRATEGY_COLOURED;
   const PlanGroup &nodes = p->groups[p->slots[0].first];
@@ -558,14 +948,13 @@ constexpr int kReduceBlocks = 2048;
 
 struct ReduceScratch {
   double *partials;
-  unsigned *tickets;
   double *out;
 };
 
-// layout: tickets (256 B), out (256 B), partials
+// layout: out (256 B), then nslots x kReduceBlocks partials
 ReduceScratch reduce_scratch(int nslots) {
-  char *base = (char *)reduct_scratch(512 + sizeof(double) * kReduceBlocks * nslots);
-  return {(double *)(base + 512), (unsigned *)base, (double *)(base + 256)};
+  char *base = (char *)reduct_scratch(256 + sizeof(double) * kReduceBlocks * nslots);
+  return {(double *)(base + 256), (double *)base};
 }
 
 template <typename T>
@@ -574,14 +963,29 @@ const double *run_update(op_set set, op_arg *a) {
   const int grid = grid_for(set->size, kReduceBlocks);
   hipLaunchKernelGGL(k_update<T>, dim3(grid), dim3(NT), 0, rt().stream,
                      (const T *)a[0].dat->data_d, (T *)a[1].dat->data_d, (T *)a[2].dat->data_d,
-                     (const T *)a[3].dat->data_d, set->size, s.partials, s.tickets, s.out);
+                     (const T *)a[3].dat->data_d, set->size, s.partials);
+  hipLaunchKernelGGL(k_fold<1>, dim3(1), dim3(NT), 0, rt().stream, (const double *)s.partials,
+                     grid, 0u, s.out, rt().deferred ? rt().d_log : nullptr, rt().d_log_count);
   check_launch("update");
   return s.out;
 }
 
 template <typename T>
 void run_jac_res(const char *name, op_set set, int nargs, op_arg *a) {
-  const int strategy = strategy_of(name);
+  const int strategy = narrow_strategy(strategy_of(name, sizeof(T)));
+  if (strategy == OP2_STRATEGY_OWNER) {
+    Plan *p = plan_for(name, set, nargs, a, strategy, part_of(name, NT));
+    if (p->arg_slot[1] != 0 || p->arg_slot[2] != 1) fail("op2: unexpected slot layout for %s", name);
+    const PlanGroup &nodes = p->groups[0];
+    const size_t lds = sizeof(T) * ((size_t)nodes.max_count + p->max_positions);
+    allow_lds(k_jac_res_owner<T>, lds);
+    hipLaunchKernelGGL(k_jac_res_owner<T>, dim3(p->nblocks), dim3(NT), lds, rt().stream,
+                       p->d_blk_off, p->nblocks, p->d_perm, nodes.d_off, nodes.d_list,
+                       (const ushort2 *)p->d_loc, p->d_owned_off, p->d_owned, p->d_csr_off,
+                       p->d_csr, (const T *)a[0].dat->data_d, (const T *)a[1].dat->data_d,
+                       (T *)a[2].dat->data_d, (T)gbl_read(a[3]));
+    return;
+  }
   Plan *p = plan_for(name, set, nargs, a, strategy, part_of(name, 1024));
   const bool col = strategy == OP2_STRATEGY_COLOURED;
   const PlanGroup &nodes = p->groups[0];
@@ -610,7 +1014,9 @@ const double *run_jac_update(op_set set, op_arg *a) {
   const T alpha = (T)get_const("alpha").as_double(0);
   hipLaunchKernelGGL(k_jac_update<T>, dim3(grid), dim3(NT), 0, rt().stream,
                      (const T *)a[0].dat->data_d, (T *)a[1].dat->data_d, (T *)a[2].dat->data_d,
-                     set->size, alpha, s.partials, s.tickets, s.out);
+                     set->size, alpha, s.partials);
+  hipLaunchKernelGGL(k_fold<2>, dim3(1), dim3(NT), 0, rt().stream, (const double *)s.partials,
+                     grid, 2u, s.out, rt().deferred ? rt().d_log : nullptr, rt().d_log_count);
   check_launch("update");
   return s.out;
 }
@@ -707,12 +1113,14 @@ int par_loop(const char *name, op_set set, int nargs, op_arg *args) {
   if (l == Loop::kUpdate) {
     const double *d = rb == 8 ? run_update<double>(set, args) : run_update<float>(set, args);
     timing_end(t, start);
+    if (rt().deferred) return 0;
     gbl_inc(args[4], fetch_results(d, 1)[0]);
     return 0;
   }
   if (l == Loop::kJacUpdate) {
     const double *d = rb == 8 ? run_jac_update<double>(set, args) : run_jac_update<float>(set, args);
     timing_end(t, start);
+    if (rt().deferred) return 0;
     const double *h = fetch_results(d, 2);
     gbl_inc(args[3], h[0]);
     gbl_max(args[4], h[1]);

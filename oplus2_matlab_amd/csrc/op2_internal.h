@@ -89,11 +89,27 @@ struct Plan {
   std::vector<int> colblk_off;  // [ncolors+1]
   long long verify_errors = -1;
   double mean_thread_colours = 0;
+  // owner-compute layout (OP2_STRATEGY_OWNER): the targets of the OP_INC
+  // slots are partitioned into tiles, one per block; a block holds every
+  // element that increments one of its targets (positions may repeat an
+  // element in two blocks) and, per owned target, the list of
+  // (position << 2 | inc slot) entries that increment it.
+  int owner_group = -1;
+  std::vector<int> inc_slot_ids;   // slot id of each INC slot, in INC order
+  std::vector<int> owned_off;      // [nblocks+1]
+  std::vector<int> owned;          // global target ids
+  std::vector<int> csr_off;        // [owned.size()+1]
+  std::vector<uint16_t> csr;       // (position in block << 2) | inc slot index
+  int max_positions = 0, max_owned = 0;
 
   // device copies
   int *d_blk_off = nullptr, *d_perm = nullptr, *d_col_off = nullptr,
       *d_col_base = nullptr, *d_blk_ncol = nullptr, *d_blkmap = nullptr;
   uint16_t *d_loc = nullptr;
+  std::vector<int> ids;            // [positions * slot_stride] global targets (gather form)
+  int *d_ids = nullptr;
+  int *d_owned_off = nullptr, *d_owned = nullptr, *d_csr_off = nullptr;
+  uint16_t *d_csr = nullptr;
 
   // the op_plan view returned by op_plan_core / op_plan_get
   op_plan view{};
@@ -135,7 +151,18 @@ struct Runtime {
   size_t reduct_bytes = 0;
   double *h_result = nullptr;  // pinned
   bool timing_enabled = true;
+  // deferred reductions: results appended to a device log instead of being
+  // written back to the caller's variable
+  bool deferred = false;
+  double *d_log = nullptr;
+  unsigned *d_log_count = nullptr;
+  unsigned long long log_read = 0;
+  // graph capture of loop sequences on the runtime stream
+  bool capturing = false;
+  std::vector<hipGraphExec_t> graphs;
 };
+
+constexpr unsigned kLogCap = 1u << 16;  // power of two
 
 Runtime &rt();
 void ensure_device();
@@ -168,6 +195,7 @@ void timing_drain();
 
 // scratch for reductions: at least `bytes` of device memory, 256-aligned
 void *reduct_scratch(size_t bytes);
+void ensure_reduction_log();
 
 // loop executor (op2_exec.hip)
 int par_loop(const char *name, op_set set, int nargs, op_arg *args);

@@ -182,31 +182,4 @@ __device__ __forceinline__ double block_reduce(double v, double *lds) {
   return v;
 }
 
-// Grid combine: each block stores its partial; the last block to arrive
-// folds the partials in block order (deterministic), writes out[slot] and
-// resets the ticket for the next launch.
-template <int OP, int NT>
-__device__ __forceinline__ void grid_reduce(double v, double *partials, unsigned *ticket,
-                                            double *out, double *lds, bool *is_last) {
-  v = block_reduce<OP, NT>(v, lds);
-  if (threadIdx.x == 0) {
-    partials[blockIdx.x] = v;
-    __threadfence();
-    unsigned t = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    *is_last = (t == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (!*is_last) return;
-  __threadfence();
-  double acc = red_identity<OP>();
-  for (int i = threadIdx.x; i < (int)gridDim.x; i += NT)
-    acc = red_apply<OP>(acc, __hip_atomic_load(partials + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-  __syncthreads();
-  acc = block_reduce<OP, NT>(acc, lds);
-  if (threadIdx.x == 0) {
-    *out = acc;
-    __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
-}
-
 }  // namespace op2

@@ -57,12 +57,23 @@ void Plan::upload() {
     d_blk_ncol = (int *)up(blk_ncol.data(), sizeof(int) * blk_ncol.size());
   }
   d_blkmap = (int *)up(blkmap.data(), sizeof(int) * blkmap.size());
+  if (!ids.empty()) d_ids = (int *)up(ids.data(), sizeof(int) * ids.size());
+  if (!owned_off.empty()) {
+    d_owned_off = (int *)up(owned_off.data(), sizeof(int) * owned_off.size());
+    d_owned = (int *)up(owned.data(), sizeof(int) * std::max<size_t>(owned.size(), 1));
+    d_csr_off = (int *)up(csr_off.data(), sizeof(int) * csr_off.size());
+    d_csr = (uint16_t *)up(csr.data(), sizeof(uint16_t) * std::max<size_t>(csr.size(), 1));
+  }
 }
 
 void Plan::release() {
   for (void *p : {(void *)d_blk_off, (void *)d_perm, (void *)d_col_off, (void *)d_col_base,
-                  (void *)d_blk_ncol, (void *)d_blkmap, (void *)d_loc})
+                  (void *)d_blk_ncol, (void *)d_blkmap, (void *)d_loc, (void *)d_owned_off,
+                  (void *)d_owned, (void *)d_csr_off, (void *)d_csr, (void *)d_ids})
     dev_free(p);
+  d_ids = nullptr;
+  d_owned_off = d_owned = d_csr_off = nullptr;
+  d_csr = nullptr;
   for (auto &g : groups) {
     dev_free(g.d_off);
     dev_free(g.d_list);
@@ -102,6 +113,146 @@ static int greedy_colour(int nitems, std::vector<int> &colour,
     base += 32;
   }
   return ncol;
+}
+
+
+// Owner-compute layout.  The distinct targets of the OP_INC slots (all
+// through one map) are cut into tiles of about `tile` targets: when an
+// element increments two or more targets, tiles grow breadth-first over that
+// adjacency (compact tiles, few cut elements); otherwise they are runs of
+// ascending target ids.  Tiles are seeded in ascending id order, so tile
+// numbers follow the user's numbering.
+static void build_owner_layout(Plan &P, const std::vector<int> &inc_slots, int tile) {
+  const int n = P.set->size;
+  const int og = P.slots[inc_slots[0]].first;
+  for (int s : inc_slots)
+    if (P.slots[s].first != og)
+      fail("op2: owner-compute needs every OP_INC argument of %s through one map", P.loop.c_str());
+  P.owner_group = og;
+  P.inc_slot_ids = inc_slots;
+  op_map om = P.groups[og].map;
+  const int nt = om->to->size;
+  const int ni = (int)inc_slots.size();
+  auto tgt = [&](int e, int k) { return target(om, P.slots[inc_slots[k]].second, e); };
+
+  std::vector<int> tile_of(nt, -2);  // -2: never incremented
+  for (int e = 0; e < n; e++)
+    for (int k = 0; k < ni; k++) tile_of[tgt(e, k)] = -1;
+
+  int ntiles = 0;
+  if (ni >= 2) {
+    // adjacency between targets incremented by one element
+    std::vector<int> deg(nt + 1, 0);
+    for (int e = 0; e < n; e++)
+      for (int k = 0; k < ni; k++)
+        for (int l = 0; l < ni; l++)
+          if (k != l && tgt(e, k) != tgt(e, l)) deg[tgt(e, k) + 1]++;
+    for (int t = 0; t < nt; t++) deg[t + 1] += deg[t];
+    std::vector<int> adj(deg[nt]);
+    std::vector<int> fill(deg.begin(), deg.end() - 1);
+    for (int e = 0; e < n; e++)
+      for (int k = 0; k < ni; k++)
+        for (int l = 0; l < ni; l++)
+          if (k != l && tgt(e, k) != tgt(e, l)) adj[fill[tgt(e, k)]++] = tgt(e, l);
+    std::vector<int> queue(nt);
+    for (int seed = 0; seed < nt; seed++) {
+      if (tile_of[seed] != -1) continue;
+      int head = 0, tail = 0, count = 1;
+      tile_of[seed] = ntiles;
+      queue[tail++] = seed;
+      while (head < tail && count < tile) {
+        const int u = queue[head++];
+        for (int j = deg[u]; j < deg[u + 1] && count < tile; j++) {
+          const int v = adj[j];
+          if (tile_of[v] != -1) continue;
+          tile_of[v] = ntiles;
+          queue[tail++] = v;
+          count++;
+        }
+      }
+      ntiles++;
+    }
+  } else {
+    int count = 0;
+    for (int t = 0; t < nt; t++) {
+      if (tile_of[t] != -1) continue;
+      tile_of[t] = ntiles;
+      if (++count == tile) {
+        count = 0;
+        ntiles++;
+      }
+    }
+    if (count) ntiles++;
+  }
+  P.nblocks = ntiles;
+
+  // owned targets per tile (ascending)
+  P.owned_off.assign(ntiles + 1, 0);
+  for (int t = 0; t < nt; t++)
+    if (tile_of[t] >= 0) P.owned_off[tile_of[t] + 1]++;
+  for (int b = 0; b < ntiles; b++) P.owned_off[b + 1] += P.owned_off[b];
+  P.owned.resize(P.owned_off[ntiles]);
+  std::vector<int> local(nt, -1);
+  {
+    std::vector<int> fill(P.owned_off.begin(), P.owned_off.end() - 1);
+    for (int t = 0; t < nt; t++)
+      if (tile_of[t] >= 0) {
+        local[t] = fill[tile_of[t]] - P.owned_off[tile_of[t]];
+        P.owned[fill[tile_of[t]]++] = t;
+      }
+  }
+  for (int b = 0; b < ntiles; b++)
+    P.max_owned = std::max(P.max_owned, P.owned_off[b + 1] - P.owned_off[b]);
+
+  // elements of each tile: those incrementing one of its targets (ascending)
+  std::vector<int> cnt(ntiles + 1, 0);
+  for (int e = 0; e < n; e++)
+    for (int k = 0; k < ni; k++) {
+      const int b = tile_of[tgt(e, k)];
+      bool first = true;
+      for (int l = 0; l < k; l++) first &= tile_of[tgt(e, l)] != b;
+      if (first) cnt[b + 1]++;
+    }
+  for (int b = 0; b < ntiles; b++) cnt[b + 1] += cnt[b];
+  P.blk_off = cnt;
+  P.perm.resize(cnt[ntiles]);
+  {
+    std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+    for (int e = 0; e < n; e++)
+      for (int k = 0; k < ni; k++) {
+        const int b = tile_of[tgt(e, k)];
+        bool first = true;
+        for (int l = 0; l < k; l++) first &= tile_of[tgt(e, l)] != b;
+        if (first) P.perm[fill[b]++] = e;
+      }
+  }
+  P.max_block = 0;
+  for (int b = 0; b < ntiles; b++) P.max_block = std::max(P.max_block, cnt[b + 1] - cnt[b]);
+  P.max_positions = P.max_block;
+  if (P.max_block >= (1 << 14))
+    fail("op2: owner tile of %s holds %d elements (limit 16383); reduce the part size",
+         P.loop.c_str(), P.max_block);
+
+  // per owned target: the (position, inc slot) pairs that increment it
+  P.csr_off.assign(P.owned.size() + 1, 0);
+  for (int b = 0; b < ntiles; b++)
+    for (int pos = cnt[b]; pos < cnt[b + 1]; pos++)
+      for (int k = 0; k < ni; k++) {
+        const int t = tgt(P.perm[pos], k);
+        if (tile_of[t] == b) P.csr_off[P.owned_off[b] + local[t] + 1]++;
+      }
+  for (size_t i = 0; i < P.owned.size(); i++) P.csr_off[i + 1] += P.csr_off[i];
+  P.csr.resize(P.csr_off.back());
+  {
+    std::vector<int> fill(P.csr_off.begin(), P.csr_off.end() - 1);
+    for (int b = 0; b < ntiles; b++)
+      for (int pos = cnt[b]; pos < cnt[b + 1]; pos++)
+        for (int k = 0; k < ni; k++) {
+          const int t = tgt(P.perm[pos], k);
+          if (tile_of[t] == b)
+            P.csr[fill[P.owned_off[b] + local[t]]++] = (uint16_t)(((pos - cnt[b]) << 2) | k);
+        }
+  }
 }
 
 static Plan *build(const PlanRequest &req) {
@@ -155,10 +306,16 @@ static Plan *build(const PlanRequest &req) {
   P.slot_stride = nslots == 1 ? 1 : nslots == 2 ? 2 : 4;
 
   const int bsize = req.part_size > 0 ? req.part_size : 512;
-  P.nblocks = (n + bsize - 1) / bsize;
-  P.blk_off.resize(P.nblocks + 1);
-  for (int b = 0; b <= P.nblocks; b++) P.blk_off[b] = std::min(n, b * bsize);
-  P.max_block = std::min(n, bsize);
+  const bool owner = req.strategy == OP2_STRATEGY_OWNER || req.strategy == OP2_STRATEGY_OWNER_GATHER;
+  if (owner) {
+    if (inc_slots.empty()) fail("op2: owner-compute plan for %s without OP_INC arguments", req.loop.c_str());
+    build_owner_layout(P, inc_slots, bsize);
+  } else {
+    P.nblocks = (n + bsize - 1) / bsize;
+    P.blk_off.resize(P.nblocks + 1);
+    for (int b = 0; b <= P.nblocks; b++) P.blk_off[b] = std::min(n, b * bsize);
+    P.max_block = std::min(n, bsize);
+  }
 
   const int ng = (int)P.groups.size();
   std::vector<std::vector<uint32_t>> mask(ng);
@@ -208,9 +365,10 @@ static Plan *build(const PlanRequest &req) {
     P.mean_thread_colours = total / P.nblocks;
   }
   auto elem = [&](int pos) { return P.perm.empty() ? pos : P.perm[pos]; };
+  const size_t npos = P.blk_off[P.nblocks];
 
   // per block and group: distinct targets (ascending) and local indices
-  P.loc.assign((size_t)n * P.slot_stride, 0);
+  P.loc.assign(npos * P.slot_stride, 0);
   std::vector<int> buf;
   for (int b = 0; b < P.nblocks; b++) {
     const int lo = P.blk_off[b], hi = P.blk_off[b + 1];
@@ -219,7 +377,7 @@ static Plan *build(const PlanRequest &req) {
       buf.clear();
       for (int s = 0; s < nslots; s++)
         if (P.slots[s].first == g)
-          for (int e = lo; e < hi; e++) buf.push_back(target(G.map, P.slots[s].second, e));
+          for (int pos = lo; pos < hi; pos++) buf.push_back(target(G.map, P.slots[s].second, elem(pos)));
       std::sort(buf.begin(), buf.end());
       buf.erase(std::unique(buf.begin(), buf.end()), buf.end());
       G.off[b] = (int)G.list.size();
@@ -262,8 +420,16 @@ static Plan *build(const PlanRequest &req) {
     std::vector<int> fill(P.colblk_off.begin(), P.colblk_off.end() - 1);
     for (int b = 0; b < P.nblocks; b++) P.blkmap[fill[P.blkcol[b]]++] = b;
   }
+  if (req.strategy == OP2_STRATEGY_OWNER_GATHER) {
+    P.ids.assign(npos * P.slot_stride, 0);
+    for (size_t pos = 0; pos < npos; pos++)
+      for (int s = 0; s < nslots; s++)
+        P.ids[pos * P.slot_stride + s] =
+            target(P.groups[P.slots[s].first].map, P.slots[s].second, elem((int)pos));
+  }
   // kernels read perm only for direct dats, and only if elements moved
   P.need_perm = has_direct_dat && !P.perm.empty();
+  if (owner) P.mean_thread_colours = 0;
 
   P.verify_errors = verify_plan(P);
   if (P.verify_errors != 0)
@@ -298,7 +464,84 @@ static Plan *build(const PlanRequest &req) {
 // of a block are a permutation of its elements; no two elements of one
 // thread colour in a block, and no two blocks of one block colour, share an
 // OP_INC target; colour ranges tile each block.
+// Owner-compute plans: every increment (element, OP_INC slot) is applied
+// exactly once, by the block that owns its target, and the owned sets
+// partition the incremented targets.
+static long long verify_owner(const Plan &P) {
+  long long err = 0;
+  const int n = P.set->size;
+  const int nslots = (int)P.slots.size();
+  const int ni = (int)P.inc_slot_ids.size();
+  op_map om = P.groups[P.owner_group].map;
+  if ((int)P.blk_off.size() != P.nblocks + 1 || P.blk_off[0] != 0) err++;
+  if ((int)P.owned_off.size() != P.nblocks + 1) return err + 1;
+  for (int b = 0; b < P.nblocks; b++) {
+    for (int pos = P.blk_off[b]; pos < P.blk_off[b + 1]; pos++) {
+      const int e = P.perm[pos];
+      if (e < 0 || e >= n || (pos > P.blk_off[b] && e <= P.perm[pos - 1])) {
+        err++;
+        continue;
+      }
+      for (int s = 0; s < nslots; s++) {
+        const PlanGroup &G = P.groups[P.slots[s].first];
+        const int l = P.loc[(size_t)pos * P.slot_stride + s];
+        if (l >= G.off[b + 1] - G.off[b]) {
+          err++;
+          continue;
+        }
+        err += G.list[G.off[b] + l] != target(G.map, P.slots[s].second, e);
+      }
+    }
+    for (int i = P.owned_off[b] + 1; i < P.owned_off[b + 1]; i++) err += P.owned[i] <= P.owned[i - 1];
+  }
+  for (const auto &G : P.groups)
+    for (int b = 0; b < P.nblocks; b++)
+      for (int k = G.off[b] + 1; k < G.off[b + 1]; k++) err += G.list[k] <= G.list[k - 1];
+  // owned sets partition the incremented targets
+  std::vector<char> inc(om->to->size, 0), own(om->to->size, 0);
+  for (int e = 0; e < n; e++)
+    for (int k = 0; k < ni; k++) inc[target(om, P.slots[P.inc_slot_ids[k]].second, e)] = 1;
+  for (int t : P.owned) {
+    if (t < 0 || t >= om->to->size || own[t]) {
+      err++;
+      continue;
+    }
+    own[t] = 1;
+  }
+  for (int t = 0; t < om->to->size; t++) err += inc[t] != own[t];
+  // every (element, inc slot) exactly once, at the owner of its target
+  std::vector<long long> seen;
+  seen.reserve((size_t)n * ni);
+  for (int b = 0; b < P.nblocks; b++)
+    for (int i = P.owned_off[b]; i < P.owned_off[b + 1]; i++)
+      for (int j = P.csr_off[i]; j < P.csr_off[i + 1]; j++) {
+        const int pos = P.blk_off[b] + (P.csr[j] >> 2), k = P.csr[j] & 3;
+        if (pos >= P.blk_off[b + 1] || k >= ni) {
+          err++;
+          continue;
+        }
+        const int e = P.perm[pos];
+        err += target(om, P.slots[P.inc_slot_ids[k]].second, e) != P.owned[i];
+        seen.push_back((long long)e * ni + k);
+      }
+  std::sort(seen.begin(), seen.end());
+  if ((long long)seen.size() != (long long)n * ni)
+    err++;
+  else
+    for (size_t i = 0; i < seen.size(); i++) err += seen[i] != (long long)i;
+  return err;
+}
+
 long long verify_plan(const Plan &P) {
+  if (P.strategy == OP2_STRATEGY_OWNER || P.strategy == OP2_STRATEGY_OWNER_GATHER) {
+    long long err = verify_owner(P);
+    const int nslots = (int)P.slots.size();
+    for (size_t pos = 0; pos < P.ids.size() / std::max(P.slot_stride, 1); pos++)
+      for (int s = 0; s < nslots; s++)
+        err += P.ids[pos * P.slot_stride + s] !=
+               target(P.groups[P.slots[s].first].map, P.slots[s].second, P.perm[pos]);
+    return err;
+  }
   long long err = 0;
   const int n = P.set->size;
   const int nslots = (int)P.slots.size();
@@ -431,7 +674,8 @@ op_plan *op2_plan_get(char const *name, op_set set, int part_size, int nargs, op
 int op2_plan_summary_get(char const *name, op_set set, int nargs, op_arg *args, int strategy,
                          int part_size, op2_plan_summary *out) {
   try {
-    if (strategy != OP2_STRATEGY_COLOURED && strategy != OP2_STRATEGY_ATOMIC)
+    if (strategy != OP2_STRATEGY_COLOURED && strategy != OP2_STRATEGY_ATOMIC &&
+        strategy != OP2_STRATEGY_OWNER && strategy != OP2_STRATEGY_OWNER_GATHER)
       fail("op2: plan summary needs an explicit strategy");
     PlanRequest req{name, set, nargs, args, strategy, part_size};
     Plan *p = get_plan(req);
@@ -455,7 +699,7 @@ int op2_plan_export(char const *name, op_set set, int nargs, op_arg *args, int s
                     int part_size, int *perm, int *thrcol, int *blk_off, int *blkcol) {
   try {
     if (strategy != OP2_STRATEGY_COLOURED && strategy != OP2_STRATEGY_ATOMIC)
-      fail("op2: plan export needs an explicit strategy");
+      fail("op2: plan export needs the coloured or atomic strategy");
     PlanRequest req{name, set, nargs, args, strategy, part_size};
     Plan *p = get_plan(req);
     const int n = set->size;

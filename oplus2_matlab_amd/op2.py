@@ -26,7 +26,7 @@ OP_ARG_GBL, OP_ARG_DAT = 0, 1
 OP_ID = None
 OP_GBL = None
 
-STRATEGY_AUTO, STRATEGY_COLOURED, STRATEGY_ATOMIC = 0, 1, 2
+STRATEGY_AUTO, STRATEGY_COLOURED, STRATEGY_ATOMIC, STRATEGY_DIRECT, STRATEGY_OWNER, STRATEGY_OWNER_GATHER = 0, 1, 2, 3, 4, 5
 ERRORS_EXIT, ERRORS_RETURN = 0, 1
 
 _NP_TYPE = {"double": np.float64, "float": np.float32, "int": np.int32,
@@ -107,6 +107,12 @@ SIGNATURES = {
                                        C.c_int, C.POINTER(PlanSummary)]),
     "op2_plan_export": (C.c_int, [C.c_char_p, _P, C.c_int, C.POINTER(OpArg), C.c_int, C.c_int,
                                   _P, _P, _P, _P]),
+    "op2_set_deferred_reductions": (C.c_int, [C.c_int]),
+    "op2_reduction_log": (C.c_int, [_P, C.c_int]),
+    "op2_graph_begin": (C.c_int, []),
+    "op2_graph_end": (C.c_int, []),
+    "op2_graph_launch": (C.c_int, [C.c_int, C.c_int]),
+    "op2_graph_destroy": (C.c_int, [C.c_int]),
 }
 
 _lib = None
@@ -309,6 +315,41 @@ def sync() -> None:
 
 def stream() -> int:
     return _call("op2_stream")
+
+
+def set_deferred_reductions(on: bool) -> bool:
+    """Reductions go to the device log instead of the caller's array."""
+    return bool(_call("op2_set_deferred_reductions", int(on)))
+
+
+def reduction_log(max_entries: int = 1 << 16) -> np.ndarray:
+    """Pending deferred-reduction results, oldest first (waits for the GPU)."""
+    out = np.zeros(max_entries, np.float64)
+    n = _call("op2_reduction_log", out.ctypes.data, max_entries)
+    if n < 0:
+        raise OP2Error(lib().op2_last_error().decode())
+    return out[:n].copy()
+
+
+def graph_begin() -> None:
+    if _call("op2_graph_begin") != 0:
+        raise OP2Error(lib().op2_last_error().decode())
+
+
+def graph_end() -> int:
+    g = _call("op2_graph_end")
+    if g < 0:
+        raise OP2Error(lib().op2_last_error().decode())
+    return g
+
+
+def graph_launch(gid: int, count: int = 1) -> None:
+    if _call("op2_graph_launch", gid, count) != 0:
+        raise OP2Error(lib().op2_last_error().decode())
+
+
+def graph_destroy(gid: int) -> None:
+    _call("op2_graph_destroy", gid)
 
 
 def plan_summary(name: str, set_: Set, args, strategy: int = STRATEGY_COLOURED,

@@ -19,7 +19,8 @@ from oracle import oracle as O
 pytestmark = pytest.mark.gpu
 
 TOL64 = 1e-12
-STRATS = [op2.STRATEGY_ATOMIC, op2.STRATEGY_COLOURED]
+STRATS = [op2.STRATEGY_ATOMIC, op2.STRATEGY_COLOURED, op2.STRATEGY_DIRECT, op2.STRATEGY_OWNER,
+          op2.STRATEGY_OWNER_GATHER]
 
 
 def note(msg):
@@ -145,7 +146,7 @@ def test_airfoil_fp32_form(runtime):
 @pytest.mark.parametrize("strategy", STRATS)
 def test_jacobi_matches_reference(runtime, golden_dir, strategy):
     note(f"jacobi fp64 64x64 random, 200 iterations, strategy={strategy}")
-    configure(strategy, 1024)
+    configure(strategy, 1024 if strategy != op2.STRATEGY_OWNER else 256)
     fx = np.load(os.path.join(golden_dir, "jacobi_64_random_s1_f64.npz"))
     prob = M.jacobi_grid(64, seed=1)
     hs, hm, u, du = AF.run_jacobi(prob, int(fx["niter"]), 0.25, -0.25, "double")
@@ -170,8 +171,10 @@ def test_jac1_known_answer(runtime):
 # full size: properties and one-call parity at 2.88M cells
 
 @pytest.mark.parametrize("ordering", ["cartesian", "random"])
-def test_full_size_res_calc(runtime, ordering):
-    note(f"res_calc fp64 2400x1200 {ordering}: one call vs oracle, conservation")
+@pytest.mark.parametrize("strategy", [op2.STRATEGY_ATOMIC, op2.STRATEGY_OWNER])
+def test_full_size_res_calc(runtime, ordering, strategy):
+    note(f"res_calc fp64 2400x1200 {ordering} strategy={strategy}: one call vs oracle")
+    configure(strategy, 256)
     mesh = M.o_mesh(2400, 1200, ordering, seed=1)
     c = M.flow_constants().as_array()
     rng = np.random.Generator(np.random.PCG64(5))
@@ -198,3 +201,74 @@ def test_unknown_loop_is_an_error(runtime):
     d = op2.op_decl_dat(s, 1, "double", np.zeros(4), "d")
     with pytest.raises(op2.OP2Error, match="no device kernel"):
         op2.op_par_loop(None, "my_loop", s, op2.op_arg_dat(d, -1, None, 1, "double", op2.OP_RW))
+
+
+# ---------------------------------------------------------------------------
+# deferred reductions and hipGraph replay
+
+@pytest.mark.parametrize("strategy", STRATS)
+def test_airfoil_graph_replays_match_reference(runtime, golden_dir, strategy):
+    """One iteration captured as a hipGraph and replayed, with eager
+    iterations and host synchronisation between the replays; every
+    iteration's rms must match the reference history."""
+    note(f"airfoil fp64 24x12 graph replay, strategy={strategy}")
+    configure(strategy, 64)
+    fx = np.load(os.path.join(golden_dir, "airfoil_24x12_random_s1.npz"))
+    mesh = M.o_mesh(24, 12, "random", seed=1)
+    a = AF.Airfoil(mesh, fx["f64_consts"], "double")
+    ref = fx["f64_seq_rms"]
+    hist = [a.iterate()]                       # eager: builds plans
+    gid = a.capture_iteration()
+    hist += list(a.run_graph(gid, 5))          # replays 1-5
+    for _ in range(2):                         # eager, deferred, between replays
+        a.iterate()
+    log = op2.reduction_log()
+    hist += list(np.sqrt(log[1::2] / mesh.ncell))
+    op2.sync()
+    hist += list(a.run_graph(gid, int(fx["niter"]) - len(hist)))
+    op2.set_deferred_reductions(False)
+    hist = np.array(hist)
+    assert hist.size == ref.size
+    assert np.max(np.abs(hist - ref) / ref) < TOL64
+    q, _, _, _ = a.fetch()
+    assert relmax(q, fx["f64_seq_q"]) < TOL64
+
+
+@pytest.mark.parametrize("strategy", [op2.STRATEGY_COLOURED, op2.STRATEGY_OWNER])
+def test_jacobi_graph_replays_match_reference(runtime, golden_dir, strategy):
+    note(f"jacobi fp64 graph replay strategy={strategy}")
+    configure(strategy, 1024 if strategy == op2.STRATEGY_COLOURED else 256)
+    fx = np.load(os.path.join(golden_dir, "jacobi_64_random_s1_f64.npz"))
+    prob = M.jacobi_grid(64, seed=1)
+    j = AF.Jacobi(prob, 0.25, "double")
+    s0, m0 = j.iterate(-0.25)
+    op2.set_deferred_reductions(True)
+    op2.graph_begin()
+    j.iterate(-0.25)
+    gid = op2.graph_end()
+    n = int(fx["niter"]) - 1
+    op2.graph_launch(gid, 100)
+    log1 = op2.reduction_log()
+    op2.graph_launch(gid, n - 100)
+    log = np.concatenate([log1, op2.reduction_log()])
+    op2.set_deferred_reductions(False)
+    sums = np.concatenate([[s0], log[0::2]])
+    maxs = np.concatenate([[m0], log[1::2]])
+    assert relmax(sums, fx["seq_u_sum"]) < TOL64
+    assert relmax(maxs, fx["seq_u_max"]) < TOL64
+    op2.op_fetch_data(j.p_u)
+    assert relmax(j.u, fx["seq_u"]) < TOL64
+
+
+def test_graph_capture_needs_prepared_loops(runtime):
+    """A loop whose plan does not exist yet cannot be captured (its upload
+    would become part of the graph): the runtime refuses with an error."""
+    mesh = M.o_mesh(24, 12, "random", seed=1)
+    a = AF.Airfoil(mesh, M.flow_constants().as_array(), "double")
+    configure(op2.STRATEGY_COLOURED, 64)
+    op2.set_deferred_reductions(True)
+    op2.graph_begin()
+    with pytest.raises(op2.OP2Error, match="graph capture"):
+        a.res_calc()
+    op2.graph_end()
+    op2.set_deferred_reductions(False)

@@ -136,3 +136,29 @@ def test_plan_is_cached(host_runtime):
     assert C.c_int.in_dll(L, "OP_plan_index").value == n0
     op2.plan_summary("res_calc", d["edges"], d["res_args"], op2.STRATEGY_ATOMIC, 64)
     assert C.c_int.in_dll(L, "OP_plan_index").value == n0 + 1
+
+
+@pytest.mark.parametrize("ordering", ["cartesian", "random", "rcm"])
+@pytest.mark.parametrize("tile", [32, 100, 256])
+def test_owner_plans(host_runtime, ordering, tile):
+    """Owner-compute plans: the runtime's verifier checks that every
+    increment is applied once, by the owner of its target."""
+    mesh = M.o_mesh(60, 30, ordering, seed=2)
+    d = declare(mesh)
+    s = op2.plan_summary("res_calc", d["edges"], d["res_args"], op2.STRATEGY_OWNER, tile)
+    assert s["verify_errors"] == 0
+    # every cell is owned once: about ncell / tile tiles, each within the limit
+    assert s["nblocks"] >= mesh.ncell // tile
+    assert s["max_block"] <= 3 * tile + 8
+    b = op2.plan_summary("bres_calc", d["bedges"], d["bres_args"], op2.STRATEGY_OWNER, tile)
+    assert b["verify_errors"] == 0
+
+
+def test_owner_tiles_are_compact(host_runtime):
+    """Breadth-first tiles on a cartesian mesh cut few edges: elements per
+    tile stay well below the 3x of a one-row strip."""
+    mesh = M.o_mesh(240, 120, "cartesian")
+    d = declare(mesh)
+    s = op2.plan_summary("res_calc", d["edges"], d["res_args"], op2.STRATEGY_OWNER, 256)
+    assert s["verify_errors"] == 0
+    assert s["max_block"] < 2.5 * 256
