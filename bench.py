@@ -36,7 +36,8 @@ sys.path.insert(0, ROOT)
 
 METRIC = "airfoil cell-updates/s & res_calc HBM GB/s (FP64 vs FP32 form) on 1 MI355X"
 HBM_PEAK_GBS = 8000.0
-STRATEGIES = {"auto": 0, "coloured": 1, "atomic": 2, "direct": 3, "owner": 4, "owner-gather": 5}
+STRATEGIES = {"auto": 0, "coloured": 1, "atomic": 2, "direct": 3, "owner": 4, "owner-gather": 5,
+              "tune": -1}
 
 
 def parse():
@@ -46,7 +47,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--mesh", default="2400x1200", help="NIxNJ O-mesh cells")
     ap.add_argument("--ordering", default="cartesian", choices=["cartesian", "random", "rcm"])
-    ap.add_argument("--strategy", default="auto", choices=list(STRATEGIES))
+    ap.add_argument("--strategy", default="tune", choices=list(STRATEGIES),
+                    help="res_calc strategy; 'tune' runs the runtime's autotuner first")
     ap.add_argument("--part", type=int, default=0, help="res_calc part size (0 = default)")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-form measurement")
     ap.add_argument("--accuracy-iters", type=int, default=1000,
@@ -63,9 +65,10 @@ def measure(mesh, consts, dtype, a):
 
     op2.op_init([], 0)
     try:
-        if a.strategy != "auto" or a.part:
-            op2.set_loop_config("res_calc", STRATEGIES[a.strategy], a.part, 0)
+        if a.strategy not in ("auto", "tune") or a.part:
+            op2.set_loop_config("res_calc", max(STRATEGIES[a.strategy], 0), a.part, 0)
         af = AF.Airfoil(mesh, consts, dtype)
+        tuned = af.autotune() if a.strategy == "tune" else None
         for _ in range(a.warmup):
             af.iterate()
         op2.sync()
@@ -95,7 +98,8 @@ def measure(mesh, consts, dtype, a):
     finally:
         op2.op_exit()
     return {"value": mesh.ncell * a.steps / dt_graph, "ms_per_step": 1e3 * dt_graph / a.steps,
-            "eager_ms_per_step": 1e3 * dt_eager / a.steps, "loops": loops, "rms_last": rms_last}
+            "eager_ms_per_step": 1e3 * dt_eager / a.steps, "loops": loops, "rms_last": rms_last,
+            "tuned": tuned}
 
 
 def accuracy(mesh, consts, iters):
@@ -174,7 +178,8 @@ def main():
                                f"{mesh.nedge} edges), {a.ordering} ordering, iteration "
                                f"replayed as a hipGraph",
                    "cells": mesh.ncell, "edges": mesh.nedge, "ordering": a.ordering,
-                   "strategy": a.strategy, "part_size": a.part or None},
+                   "strategy": a.strategy, "part_size": a.part or None,
+                   "tuned": r64["tuned"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "res_calc", "bytes_per_launch": rc["bytes_per_call"],
@@ -191,6 +196,7 @@ def main():
                                "speedup_vs_fp64": r32["value"] / r64["value"],
                                "res_calc_GBs": r32["loops"]["res_calc"]["GBs"],
                                "res_calc_us": r32["loops"]["res_calc"]["us_per_call"],
+                               "tuned": r32["tuned"],
                                "loops": r32["loops"]}
         if a.accuracy_iters > 0:
             result["fp32_form"]["accuracy"] = accuracy(mesh, consts, a.accuracy_iters)

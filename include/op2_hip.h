@@ -154,6 +154,15 @@ void op2_clear_error(void);
 int op2_set_loop_config(char const *name, int strategy, int part_size,
                         int block_size);
 
+/* Autotuner: runs the loop (with these arguments, on the live data) under
+ * each candidate strategy and part size, reps times each, keeps the fastest
+ * as the loop's configuration and reports it.  Dats the loop writes are
+ * saved before and restored after, so the state is unchanged.  Only
+ * indirect loops without OP_GBL results (res_calc, bres_calc, jac1 res).
+ * Returns 0 on success. */
+int op2_autotune_loop(char const *name, op_set set, int nargs, op_arg *args, int reps,
+                      int *best_strategy, int *best_part, double *best_us);
+
 /* Device seconds, call count and algorithmic bytes per call of a loop
  * (device time from HIP events on the runtime stream).  Returns 0 if the
  * loop has run. */

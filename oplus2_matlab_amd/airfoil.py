@@ -85,18 +85,27 @@ class Airfoil:
                 op_arg_dat(self.p_res, 0, self.pecell, 4, t, OP_INC),
                 op_arg_dat(self.p_res, 1, self.pecell, 4, t, OP_INC))
 
+    def bres_calc_args(self):
+        t = self.t
+        return (op_arg_dat(self.p_x, 0, self.pbedge, 2, t, OP_READ),
+                op_arg_dat(self.p_x, 1, self.pbedge, 2, t, OP_READ),
+                op_arg_dat(self.p_q, 0, self.pbecell, 4, t, OP_READ),
+                op_arg_dat(self.p_adt, 0, self.pbecell, 1, t, OP_READ),
+                op_arg_dat(self.p_res, 0, self.pbecell, 4, t, OP_INC),
+                op_arg_dat(self.p_bound, -1, OP_ID, 1, "int", OP_READ))
+
+    def autotune(self, reps: int = 5) -> dict:
+        """Picks strategy and part size of res_calc and bres_calc on this
+        mesh with the runtime's autotuner (state unchanged)."""
+        self.adt_calc()
+        return {"res_calc": op2.autotune("res_calc", self.edges, self.res_calc_args(), reps),
+                "bres_calc": op2.autotune("bres_calc", self.bedges, self.bres_calc_args(), reps)}
+
     def res_calc(self):
         op_par_loop(op2.res_calc, "res_calc", self.edges, *self.res_calc_args())
 
     def bres_calc(self):
-        t = self.t
-        op_par_loop(op2.bres_calc, "bres_calc", self.bedges,
-                    op_arg_dat(self.p_x, 0, self.pbedge, 2, t, OP_READ),
-                    op_arg_dat(self.p_x, 1, self.pbedge, 2, t, OP_READ),
-                    op_arg_dat(self.p_q, 0, self.pbecell, 4, t, OP_READ),
-                    op_arg_dat(self.p_adt, 0, self.pbecell, 1, t, OP_READ),
-                    op_arg_dat(self.p_res, 0, self.pbecell, 4, t, OP_INC),
-                    op_arg_dat(self.p_bound, -1, OP_ID, 1, "int", OP_READ))
+        op_par_loop(op2.bres_calc, "bres_calc", self.bedges, *self.bres_calc_args())
 
     def update(self) -> float:
         t = self.t

@@ -698,6 +698,14 @@ int op2_graph_destroy(int id) {
   API_END(-1)
 }
 
+int op2_autotune_loop(char const *name, op_set set, int nargs, op_arg *args, int reps,
+                      int *best_strategy, int *best_part, double *best_us) {
+  API_BEGIN
+  if (!name || !set || nargs <= 0 || !args || reps <= 0) fail("op2_autotune_loop: invalid arguments");
+  return autotune(name, set, nargs, args, reps, best_strategy, best_part, best_us);
+  API_END(-1)
+}
+
 int op2_par_loop(char const *name, op_set set, int nargs, op_arg *args) {
   API_BEGIN
   return par_loop(name, set, nargs, args);

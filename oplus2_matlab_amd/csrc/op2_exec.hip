@@ -245,7 +245,8 @@ __global__ __launch_bounds__(NT) void k_res_calc_owner(
 // Owner-compute edge loop without staging (res_calc): the same tiles, but
 // each element position carries the global ids of its nodes and cells, and
 // x, q, adt are gathered straight from global memory (the tile's records
-// stay in the L1/L2); fluxes still meet in LDS for the fixed-order sums.
+// stay in the L1/L2).  Fluxes meet in LDS, component-major (8-byte lanes,
+// no bank conflicts), for the fixed-order sums of the owned cells.
 template <typename T, int OWN>
 __global__ __launch_bounds__(NT) void k_res_calc_owner_direct(
     const int *__restrict__ blk_off, int nblk, const int4 *__restrict__ ids,
@@ -257,7 +258,7 @@ __global__ __launch_bounds__(NT) void k_res_calc_owner_direct(
   const int b = xcd_block(blockIdx.x, nblk);
   const int p0 = blk_off[b], np = blk_off[b + 1] - p0;
   const int o0 = owned_off[b], no = owned_off[b + 1] - o0;
-  T *fs = reinterpret_cast<T *>(smem);
+  T *fs = reinterpret_cast<T *>(smem);  // [4][np]
   const int tid = threadIdx.x;
   T rpre[OWN][4];
   int gown[OWN];
@@ -275,7 +276,8 @@ __global__ __launch_bounds__(NT) void k_res_calc_owner_direct(
     Vec<T>::load4(q + 4 * (size_t)g.z, q1);
     Vec<T>::load4(q + 4 * (size_t)g.w, q2);
     edge_flux(x1, x2, q1, q2, adt[g.z], adt[g.w], k, f);
-    Vec<T>::store4(fs + 4 * p, f);
+#pragma unroll
+    for (int d = 0; d < 4; d++) fs[d * np + p] = f[d];
   }
   __syncthreads();
 #pragma unroll
@@ -286,14 +288,13 @@ __global__ __launch_bounds__(NT) void k_res_calc_owner_direct(
     const int j1 = csr_off[o0 + i + 1];
     for (int c = csr_off[o0 + i]; c < j1; c++) {
       const unsigned e = csr[c];
-      T f[4];
-      Vec<T>::load4(fs + 4 * (e >> 2), f);
+      const int pos = e >> 2;
       if (e & 3u) {
 #pragma unroll
-        for (int d = 0; d < 4; d++) acc[d] -= f[d];
+        for (int d = 0; d < 4; d++) acc[d] -= fs[d * np + pos];
       } else {
 #pragma unroll
-        for (int d = 0; d < 4; d++) acc[d] += f[d];
+        for (int d = 0; d < 4; d++) acc[d] += fs[d * np + pos];
       }
     }
 #pragma unroll
@@ -312,7 +313,7 @@ __global__ __launch_bounds__(NT) void k_bres_calc_owner(
     const int *__restrict__ owned, const int *__restrict__ csr_off,
     const uint16_t *__restrict__ csr, const T *__restrict__ x, const T *__restrict__ q,
     const T *__restrict__ adt, T *__restrict__ res, const int *__restrict__ bound,
-    AirfoilConsts<T> k) {
+    const T *__restrict__ qinf_dat, const int *__restrict__ bmap, AirfoilConsts<T> k) {
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr int pad = 16 / sizeof(T);
   const int b = xcd_block(blockIdx.x, nblk);
@@ -344,7 +345,9 @@ __global__ __launch_bounds__(NT) void k_bres_calc_owner(
     Vec<T>::load2(xs + 2 * L.x, x1);
     Vec<T>::load2(xs + 2 * L.y, x2);
     Vec<T>::load4(qs + 4 * L.z, q1);
-    boundary_flux(x1, x2, q1, as[L.z], bound[perm[p0 + p]], k, f);
+    const int el = perm[p0 + p];
+    const T *qi = bmap ? qinf_dat + 4 * (size_t)bmap[el] : k.qinf;
+    boundary_flux(x1, x2, q1, as[L.z], bound[el], qi, k, f);
     Vec<T>::store4(fs + 4 * p, f);
   }
   __syncthreads();
@@ -406,7 +409,7 @@ __global__ __launch_bounds__(NT) void k_bres_calc(
     const int *__restrict__ col_off, const int *__restrict__ col_base,
     const int *__restrict__ blk_ncol, const T *__restrict__ x, const T *__restrict__ q,
     const T *__restrict__ adt, T *__restrict__ res, const int *__restrict__ bound,
-    AirfoilConsts<T> k) {
+    const T *__restrict__ qinf_dat, const int *__restrict__ bmap, AirfoilConsts<T> k) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int b = blkmap[blk_first + xcd_block(blockIdx.x, nblk)];
   const int e0 = blk_off[b], ne = blk_off[b + 1] - e0;
@@ -443,7 +446,8 @@ __global__ __launch_bounds__(NT) void k_bres_calc(
       Vec<T>::load2(xs + 2 * L.x, x1);
       Vec<T>::load2(xs + 2 * L.y, x2);
       Vec<T>::load4(qs + 4 * L.z, q1);
-      boundary_flux(x1, x2, q1, as[L.z], bound[el], k, f);
+      const T *qi = bmap ? qinf_dat + 4 * (size_t)bmap[el] : k.qinf;
+      boundary_flux(x1, x2, q1, as[L.z], bound[el], qi, k, f);
       if (COLOURED) {
 #pragma unroll
         for (int d = 0; d < 4; d++) rs[4 * L.z + d] += f[d];
@@ -496,6 +500,78 @@ __global__ __launch_bounds__(NT) void k_update(const T *__restrict__ qold, T *__
   }
   acc = block_reduce<kRedSum, NT>(acc, lds);
   if (threadIdx.x == 0) partials[blockIdx.x] = acc;
+}
+
+// Batched update: as k_update, with the squared moves summed per mesh
+// (rms[cmap[c]] += ...).  cmap is non-decreasing, so each mesh is a run of
+// cells; workgroup b covers cells [b*CH, (b+1)*CH) and, for every mesh run
+// its range meets, writes one partial (block tree over its threads, in a
+// fixed order) at slot pslot[m] + b - (first workgroup of mesh m).
+// k_fold_seg adds a mesh's partials in workgroup order.
+template <typename T, int UPC>
+__global__ __launch_bounds__(NT) void k_update_seg(const T *__restrict__ qold, T *__restrict__ q,
+                                                   T *__restrict__ res, const T *__restrict__ adt,
+                                                   int ncell, const int *__restrict__ cmap,
+                                                   const int *__restrict__ seg_off,
+                                                   const int *__restrict__ pslot,
+                                                   double *__restrict__ partials) {
+  __shared__ double lds[NT / kWave];
+  constexpr int CH = NT * UPC;
+  const int c_lo = blockIdx.x * CH;
+  const int c_hi = min(c_lo + CH, ncell);
+  double s[UPC];
+#pragma unroll
+  for (int j = 0; j < UPC; j++) {
+    const int c = c_lo + threadIdx.x + j * NT;
+    s[j] = 0.0;
+    if (c >= c_hi) continue;
+    T qo[4], r[4], qn[4];
+    Vec<T>::load4(qold + 4 * (size_t)c, qo);
+    Vec<T>::load4(res + 4 * (size_t)c, r);
+    const double adti = 1.0 / (double)adt[c];
+#pragma unroll
+    for (int d = 0; d < 4; d++) {
+      const double del = adti * (double)r[d];
+      qn[d] = (T)((double)qo[d] - del);
+      s[j] += del * del;
+      r[d] = T(0);
+    }
+    Vec<T>::store4(q + 4 * (size_t)c, qn);
+    Vec<T>::store4(res + 4 * (size_t)c, r);
+  }
+  const int m_lo = cmap[c_lo], m_hi = cmap[c_hi - 1];
+  for (int m = m_lo; m <= m_hi; m++) {
+    const int a = max(c_lo, seg_off[m]), z = min(c_hi, seg_off[m + 1]);
+    if (a >= z) continue;  // a mesh without cells (uniform across the workgroup)
+    double v = 0.0;
+#pragma unroll
+    for (int j = 0; j < UPC; j++) {
+      const int c = c_lo + threadIdx.x + j * NT;
+      if (c >= a && c < z) v += s[j];
+    }
+    v = block_reduce<kRedSum, NT>(v, lds);
+    if (threadIdx.x == 0) partials[pslot[m] + blockIdx.x - seg_off[m] / CH] = v;
+    __syncthreads();
+  }
+}
+
+// Per mesh m: out = sum of its partials in workgroup order; the dat gets
+// dat[m] += out and, with deferred reductions, the log gets out for every
+// mesh in mesh order.
+__global__ __launch_bounds__(NT) void k_fold_seg(const double *__restrict__ partials,
+                                                 const int *__restrict__ pslot, int nseg,
+                                                 double *__restrict__ dat,
+                                                 double *__restrict__ log,
+                                                 unsigned *__restrict__ log_count) {
+  __shared__ unsigned base;
+  if (log && threadIdx.x == 0) base = atomicAdd(log_count, (unsigned)nseg);
+  __syncthreads();
+  for (int m = threadIdx.x; m < nseg; m += NT) {
+    double acc = 0.0;
+    for (int i = pslot[m]; i < pslot[m + 1]; i++) acc += partials[i];
+    dat[m] += acc;
+    if (log) log[(base + m) & (kLogCap - 1)] = acc;
+  }
 }
 
 // Folds the per-workgroup partials of NSLOT reductions in workgroup order
@@ -850,9 +926,11 @@ void run_res_calc(const char *name, op_set set, int nargs, op_arg *a) {
     Plan *p = plan_for(name, set, nargs, a, strategy, part_of(name, 256));
     const PlanGroup &nodes = p->groups[p->slots[0].first];
     const PlanGroup &cells = p->groups[p->owner_group];
-    if (p->max_owned > 2 * NT) fail("op2: owner tile of %s exceeds %d targets", name, 2 * NT);
+    const int own = (p->max_owned + NT - 1) / NT;
+    if (own > 4) fail("op2: owner tile of %s exceeds %d targets", name, 4 * NT);
     const size_t lds = owner_lds<T>(p, 2, 4, 4);
-    auto kern = p->max_owned > NT ? k_res_calc_owner<T, 2> : k_res_calc_owner<T, 1>;
+    auto kern = own <= 1 ? k_res_calc_owner<T, 1>
+                         : own <= 2 ? k_res_calc_owner<T, 2> : k_res_calc_owner<T, 4>;
     allow_lds(kern, lds);
     hipLaunchKernelGGL(kern, dim3(p->nblocks), dim3(NT), lds, rt().stream,
                        p->d_blk_off, p->nblocks, nodes.d_off, nodes.d_list, cells.d_off,
@@ -864,9 +942,11 @@ void run_res_calc(const char *name, op_set set, int nargs, op_arg *a) {
   }
   if (strategy == OP2_STRATEGY_OWNER_GATHER) {
     Plan *p = plan_for(name, set, nargs, a, strategy, part_of(name, 256));
-    if (p->max_owned > 2 * NT) fail("op2: owner tile of %s exceeds %d targets", name, 2 * NT);
     const size_t lds = sizeof(T) * 4 * (size_t)p->max_positions;
-    auto kern = p->max_owned > NT ? k_res_calc_owner_direct<T, 2> : k_res_calc_owner_direct<T, 1>;
+    const int own = (p->max_owned + NT - 1) / NT;
+    if (own > 4) fail("op2: owner tile of %s exceeds %d targets", name, 4 * NT);
+    auto kern = own <= 1 ? k_res_calc_owner_direct<T, 1>
+                         : own <= 2 ? k_res_calc_owner_direct<T, 2> : k_res_calc_owner_direct<T, 4>;
     allow_lds(kern, lds);
     hipLaunchKernelGGL(kern, dim3(p->nblocks), dim3(NT), lds, rt().stream, p->d_blk_off,
                        p->nblocks, (const int4 *)p->d_ids, p->d_owned_off, p->d_owned,
@@ -906,6 +986,9 @@ int narrow_strategy(int s) {
 template <typename T>
 void run_bres_calc(const char *name, op_set set, int nargs, op_arg *a) {
   const int strategy = narrow_strategy(strategy_of(name, sizeof(T)));
+  // batched form: the free stream of each boundary edge's mesh, through a map
+  const T *qd = nargs == 7 ? (const T *)a[6].dat->data_d : nullptr;
+  const int *bm = nargs == 7 ? a[6].map->map_d : nullptr;
   if (strategy == OP2_STRATEGY_OWNER) {
     Plan *p = plan_for(name, set, nargs, a, strategy, part_of(name, NT));
     const PlanGroup &nodes = p->groups[p->slots[0].first];
@@ -917,7 +1000,7 @@ void run_bres_calc(const char *name, op_set set, int nargs, op_arg *a) {
                        cells.d_off, cells.d_list, (const ushort4 *)p->d_loc, p->d_owned_off,
                        p->d_owned, p->d_csr_off, p->d_csr, (const T *)a[0].dat->data_d,
                        (const T *)a[2].dat->data_d, (const T *)a[3].dat->data_d,
-                       (T *)a[4].dat->data_d, (const int *)a[5].dat->data_d,
+                       (T *)a[4].dat->data_d, (const int *)a[5].dat->data_d, qd, bm,
                        airfoil_consts<T>());
     return;
   }
@@ -940,7 +1023,7 @@ void run_bres_calc(const char *name, op_set set, int nargs, op_arg *a) {
                        cells.d_list, (const ushort4 *)p->d_loc, p->d_col_off, p->d_col_base,
                        p->d_blk_ncol, (const T *)a[0].dat->data_d, (const T *)a[2].dat->data_d,
                        (const T *)a[3].dat->data_d, (T *)a[4].dat->data_d,
-                       (const int *)a[5].dat->data_d, k);
+                       (const int *)a[5].dat->data_d, qd, bm, k);
   }
 }
 
@@ -968,6 +1051,62 @@ const double *run_update(op_set set, op_arg *a) {
                      grid, 0u, s.out, rt().deferred ? rt().d_log : nullptr, rt().d_log_count);
   check_launch("update");
   return s.out;
+}
+
+// Segments of a non-decreasing dim-1 map (cell -> mesh) for the batched
+// update: run offsets and the partial slots of each run, cached per map.
+struct SegPlan {
+  op_map map = nullptr;
+  int nseg = 0, nslots = 0;
+  int *d_seg_off = nullptr, *d_pslot = nullptr;
+  double *d_partials = nullptr;
+};
+
+constexpr int kSegUpc = 4;  // cells per thread in k_update_seg
+
+SegPlan &seg_plan(op_map m) {
+  static std::vector<SegPlan> cache;
+  for (auto &sp : cache)
+    if (sp.map == m) return sp;
+  const int n = m->from->size, nseg = m->to->size;
+  std::vector<int> off(nseg + 1, 0);
+  for (int c = 0; c < n; c++) {
+    if (c > 0 && m->map[c] < m->map[c - 1])
+      fail("op2: batched update needs the cell -> mesh map %s to be non-decreasing", m->name);
+    off[m->map[c] + 1]++;
+  }
+  for (int i = 0; i < nseg; i++) off[i + 1] += off[i];
+  const int ch = NT * kSegUpc;
+  std::vector<int> pslot(nseg + 1, 0);
+  for (int i = 0; i < nseg; i++)
+    pslot[i + 1] = pslot[i] + (off[i + 1] > off[i] ? (off[i + 1] - 1) / ch - off[i] / ch + 1 : 0);
+  SegPlan sp;
+  sp.map = m;
+  sp.nseg = nseg;
+  sp.nslots = pslot[nseg];
+  sp.d_seg_off = (int *)dev_alloc(sizeof(int) * off.size());
+  h2d(sp.d_seg_off, off.data(), sizeof(int) * off.size());
+  sp.d_pslot = (int *)dev_alloc(sizeof(int) * pslot.size());
+  h2d(sp.d_pslot, pslot.data(), sizeof(int) * pslot.size());
+  sp.d_partials = (double *)dev_alloc(sizeof(double) * std::max(sp.nslots, 1));
+  cache.push_back(sp);
+  return cache.back();
+}
+
+template <typename T>
+void run_update_batch(op_set set, op_arg *a) {
+  SegPlan &sp = seg_plan(a[4].map);
+  const int ch = NT * kSegUpc;
+  const int grid = (set->size + ch - 1) / ch;
+  Runtime &r = rt();
+  auto kern = k_update_seg<T, kSegUpc>;
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(NT), 0, r.stream,
+                     (const T *)a[0].dat->data_d, (T *)a[1].dat->data_d, (T *)a[2].dat->data_d,
+                     (const T *)a[3].dat->data_d, set->size, (const int *)a[4].map->map_d,
+                     (const int *)sp.d_seg_off, (const int *)sp.d_pslot, sp.d_partials);
+  hipLaunchKernelGGL(k_fold_seg, dim3(1), dim3(NT), 0, r.stream, (const double *)sp.d_partials,
+                     (const int *)sp.d_pslot, sp.nseg, (double *)a[4].dat->data_d,
+                     r.deferred ? r.d_log : nullptr, r.d_log_count);
 }
 
 template <typename T>
@@ -1021,7 +1160,9 @@ const double *run_jac_update(op_set set, op_arg *a) {
   return s.out;
 }
 
-enum class Loop { kUnknown, kSaveSoln, kAdtCalc, kResCalc, kBresCalc, kUpdate, kJacRes, kJacUpdate };
+enum class Loop {
+  kUnknown, kSaveSoln, kAdtCalc, kResCalc, kBresCalc, kUpdate, kUpdateBatch, kJacRes, kJacUpdate
+};
 
 Loop identify(const char *name, op_set set, int nargs, const op_arg *a) {
   auto is = [&](const char *n) { return !strcmp(name, n); };
@@ -1054,10 +1195,28 @@ Loop identify(const char *name, op_set set, int nargs, const op_arg *a) {
               direct(a[5], 1, OP_READ, set) && is_type(a[5], "int") && pb != pc;
     if (ok) return Loop::kBresCalc;
   }
+  // batched airfoil: each boundary edge reads its mesh's free stream
+  if (is("bres_calc") && nargs == 7 && a[0].map && a[2].map && a[6].map &&
+      a[0].map->dim == 2 && a[2].map->dim == 1 && a[6].map->dim == 1 &&
+      a[0].map->from == set && a[2].map->from == set && a[6].map->from == set) {
+    op_map pb = a[0].map, pc = a[2].map;
+    bool ok = via(a[0], pb, 0, 2, OP_READ) && via(a[1], pb, 1, 2, OP_READ) &&
+              a[1].dat == a[0].dat && via(a[2], pc, 0, 4, OP_READ) &&
+              via(a[3], pc, 0, 1, OP_READ) && via(a[4], pc, 0, 4, OP_INC) &&
+              direct(a[5], 1, OP_READ, set) && is_type(a[5], "int") && pb != pc &&
+              via(a[6], a[6].map, 0, 4, OP_READ) && a[6].map != pb && a[6].map != pc;
+    if (ok) return Loop::kBresCalc;
+  }
   if (is("update") && nargs == 5 && direct(a[0], 4, OP_READ, set) &&
       direct(a[1], 4, OP_WRITE, set) && direct(a[2], 4, OP_RW, set) &&
       direct(a[3], 1, OP_READ, set) && gbl(a[4], 1, OP_INC))
     return Loop::kUpdate;
+  // batched airfoil: the rms of each mesh, incremented through cell -> mesh
+  if (is("update") && nargs == 5 && direct(a[0], 4, OP_READ, set) &&
+      direct(a[1], 4, OP_WRITE, set) && direct(a[2], 4, OP_RW, set) &&
+      direct(a[3], 1, OP_READ, set) && a[4].map && a[4].map->from == set &&
+      a[4].map->dim == 1 && via(a[4], a[4].map, 0, 1, OP_INC) && is_type(a[4], "double"))
+    return Loop::kUpdateBatch;
   if (is("res") && nargs == 4 && direct(a[0], 1, OP_READ, set) && a[1].map &&
       a[1].map->dim == 2 && via(a[1], a[1].map, 1, 1, OP_READ) &&
       via(a[2], a[1].map, 0, 1, OP_INC) && gbl(a[3], 1, OP_READ))
@@ -1101,7 +1260,8 @@ int par_loop(const char *name, op_set set, int nargs, op_arg *args) {
   if (l == Loop::kUnknown)
     fail("op_par_loop error -- no device kernel for loop \"%s\" with this argument signature",
          name);
-  const int rb = real_bytes(nargs, args);
+  // the batched update's per-mesh rms is double whatever the state type
+  const int rb = real_bytes(l == Loop::kUpdateBatch ? 4 : nargs, args);
   if (rb != 4 && rb != 8)
     fail("op_par_loop error -- loop \"%s\" needs all its real dats to be double or all float",
          name);
@@ -1115,6 +1275,15 @@ int par_loop(const char *name, op_set set, int nargs, op_arg *args) {
     timing_end(t, start);
     if (rt().deferred) return 0;
     gbl_inc(args[4], fetch_results(d, 1)[0]);
+    return 0;
+  }
+  if (l == Loop::kUpdateBatch) {
+    if (rb == 8)
+      run_update_batch<double>(set, args);
+    else
+      run_update_batch<float>(set, args);
+    check_launch(name);
+    timing_end(t, start);
     return 0;
   }
   if (l == Loop::kJacUpdate) {
@@ -1132,6 +1301,123 @@ int par_loop(const char *name, op_set set, int nargs, op_arg *args) {
     dispatch<float>(l, name, set, nargs, args);
   check_launch(name);
   timing_end(t, start);
+  return 0;
+}
+
+// Autotuner: times candidate (strategy, part size) pairs of one loop on the
+// live mesh and keeps the fastest as the loop's configuration.  Every dat
+// the loop writes is saved before and restored after, so the state is
+// unchanged; plans of the losing candidates are freed.  Loops with global
+// reductions are not tuned (their results would be consumed).
+int autotune(const char *name, op_set set, int nargs, op_arg *args, int reps, int *best_strategy,
+             int *best_part, double *best_us) {
+  ensure_device();
+  const Loop l = identify(name, set, nargs, args);
+  if (l != Loop::kResCalc && l != Loop::kBresCalc && l != Loop::kJacRes)
+    fail("op2: loop \"%s\" cannot be autotuned (only indirect loops without global results)",
+         name);
+  // the batched update's per-mesh rms is double whatever the state type
+  const int rb = real_bytes(l == Loop::kUpdateBatch ? 4 : nargs, args);
+  if (rb != 4 && rb != 8) fail("op2: loop \"%s\" mixes real types", name);
+  for (int m = 0; m < nargs; m++)
+    if (args[m].argtype == OP_ARG_DAT && args[m].map) ensure_map_device(args[m].map);
+  Runtime &r = rt();
+  if (r.capturing) fail("op2: autotuning during graph capture");
+  // save written dats
+  std::vector<std::pair<op_dat, void *>> saved;
+  for (int m = 0; m < nargs; m++) {
+    const op_arg &a = args[m];
+    if (a.argtype != OP_ARG_DAT || a.acc == OP_READ) continue;
+    bool have = false;
+    for (auto &sv : saved) have |= sv.first == a.dat;
+    if (have) continue;
+    const size_t bytes = (size_t)a.dat->size * a.dat->set->size;
+    void *tmp = dev_alloc(bytes);
+    OP2_HIP_CHECK(hipMemcpyAsync(tmp, a.dat->data_d, bytes, hipMemcpyDeviceToDevice, r.stream));
+    saved.emplace_back(a.dat, tmp);
+  }
+  const LoopConfig keep = config_for(name);
+  const int keep_mode = r.error_mode;
+  const bool keep_timing = r.timing_enabled;
+  r.error_mode = OP2_ERRORS_RETURN;
+  r.timing_enabled = false;
+  hipEvent_t e0, e1;
+  OP2_HIP_CHECK(hipEventCreate(&e0));
+  OP2_HIP_CHECK(hipEventCreate(&e1));
+  double best = 1e300;
+  int bs = 0, bp = 0;
+  auto trial = [&](int strategy, int part) {
+    r.loop_configs[name] = LoopConfig{strategy, part, 0};
+    const size_t nplans = r.plans.size();
+    double us = 1e300;
+    try {
+      if (rb == 8)
+        dispatch<double>(l, name, set, nargs, args);
+      else
+        dispatch<float>(l, name, set, nargs, args);
+      check_launch(name);
+      OP2_HIP_CHECK(hipEventRecord(e0, r.stream));
+      for (int i = 0; i < reps; i++) {
+        if (rb == 8)
+          dispatch<double>(l, name, set, nargs, args);
+        else
+          dispatch<float>(l, name, set, nargs, args);
+      }
+      OP2_HIP_CHECK(hipEventRecord(e1, r.stream));
+      OP2_HIP_CHECK(hipEventSynchronize(e1));
+      float ms = 0;
+      OP2_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+      us = 1e3 * ms / reps;
+    } catch (const op2::Error &) {
+      (void)hipGetLastError();
+    }
+    if (OP_diags > 0) printf(" autotune %s: strategy %d part %d: %.1f us\n", name, strategy, part, us);
+    if (us < best) {
+      best = us;
+      bs = strategy;
+      bp = part;
+    }
+    // free the plans this trial built unless it is the best so far
+    while (r.plans.size() > nplans) {
+      Plan *p = r.plans.back().get();
+      if (p->strategy == bs && p->part_size == bp) break;
+      drop_plan(p);
+    }
+  };
+  const int strategies[] = {OP2_STRATEGY_OWNER_GATHER, OP2_STRATEGY_OWNER, OP2_STRATEGY_ATOMIC,
+                            OP2_STRATEGY_COLOURED};
+  for (int st : strategies) {
+    if (l != Loop::kResCalc && st == OP2_STRATEGY_OWNER_GATHER) continue;
+    trial(st, 256);
+  }
+  for (int part : {128, 512}) trial(bs, part);
+  OP2_HIP_CHECK(hipEventDestroy(e0));
+  OP2_HIP_CHECK(hipEventDestroy(e1));
+  // drop every losing plan of this loop that is still cached
+  for (size_t i = 0; i < r.plans.size();) {
+    Plan *p = r.plans[i].get();
+    if (p->loop == name && p->set == set && !(p->strategy == bs && p->part_size == bp))
+      drop_plan(p);
+    else
+      i++;
+  }
+  for (auto &sv : saved) {
+    const size_t bytes = (size_t)sv.first->size * sv.first->set->size;
+    OP2_HIP_CHECK(hipMemcpyAsync(sv.first->data_d, sv.second, bytes, hipMemcpyDeviceToDevice,
+                                 r.stream));
+  }
+  OP2_HIP_CHECK(hipStreamSynchronize(r.stream));
+  for (auto &sv : saved) dev_free(sv.second);
+  r.error_mode = keep_mode;
+  r.timing_enabled = keep_timing;
+  if (best >= 1e300) {
+    r.loop_configs[name] = keep;
+    fail("op2: no candidate configuration of loop \"%s\" ran", name);
+  }
+  r.loop_configs[name] = LoopConfig{bs, bp, 0};
+  if (best_strategy) *best_strategy = bs;
+  if (best_part) *best_part = bp;
+  if (best_us) *best_us = best;
   return 0;
 }
 

@@ -186,6 +186,8 @@ struct PlanRequest {
 };
 Plan *get_plan(const PlanRequest &req);
 long long verify_plan(const Plan &p);
+// releases a plan's device arrays and forgets it
+void drop_plan(Plan *p);
 
 // timing (op2_core.cpp)
 LoopTiming &timing_for(const std::string &name, double bytes_per_call);
@@ -199,5 +201,7 @@ void ensure_reduction_log();
 
 // loop executor (op2_exec.hip)
 int par_loop(const char *name, op_set set, int nargs, op_arg *args);
+int autotune(const char *name, op_set set, int nargs, op_arg *args, int reps, int *best_strategy,
+             int *best_part, double *best_us);
 
 }  // namespace op2

@@ -122,10 +122,11 @@ __device__ __forceinline__ void edge_flux(const T x1[2], const T x2[2], const T 
   f[3] = T(0.5) * (vol1 * (q1[3] + p1) + vol2 * (q2[3] + p2)) + mu * (q1[3] - q2[3]);
 }
 
-// airfoil/bres_calc.h:1-33: the increment of the boundary cell
+// airfoil/bres_calc.h:1-33: the increment of the boundary cell; qi is the
+// free stream (the qinf constant, or the boundary edge's mesh's in a batch)
 template <typename T>
 __device__ __forceinline__ void boundary_flux(const T x1[2], const T x2[2], const T q1[4],
-                                              T adt1, int bound,
+                                              T adt1, int bound, const T *qi,
                                               const AirfoilConsts<T> &k, T f[4]) {
   T dx = x1[0] - x2[0];
   T dy = x1[1] - x2[1];
@@ -138,7 +139,6 @@ __device__ __forceinline__ void boundary_flux(const T x1[2], const T x2[2], cons
     f[3] = T(0);
     return;
   }
-  const T *qi = k.qinf;
   T vol1 = ri * (q1[1] * dy - q1[2] * dx);
   ri = T(1) / qi[0];
   T p2 = k.gm1 * (qi[3] - T(0.5) * ri * (qi[1] * qi[1] + qi[2] * qi[2]));

@@ -431,7 +431,11 @@ static Plan *build(const PlanRequest &req) {
   P.need_perm = has_direct_dat && !P.perm.empty();
   if (owner) P.mean_thread_colours = 0;
 
+  double tv0, tv1, cpu;
+  op_timers(&cpu, &tv0);
   P.verify_errors = verify_plan(P);
+  op_timers(&cpu, &tv1);
+  if (OP_diags > 1) printf(" plan verification %.3f s\n", tv1 - tv0);
   if (P.verify_errors != 0)
     fail("op2: plan verification failed for loop %s (%lld violations)", req.loop.c_str(),
          P.verify_errors);
@@ -621,6 +625,18 @@ long long verify_plan(const Plan &P) {
   return err;
 }
 
+void drop_plan(Plan *p) {
+  Runtime &r = rt();
+  for (size_t i = 0; i < r.plans.size(); i++)
+    if (r.plans[i].get() == p) {
+      if (r.device_ready) (void)hipStreamSynchronize(r.stream);
+      p->release();
+      r.plans.erase(r.plans.begin() + i);
+      OP_plan_index = (int)r.plans.size();
+      return;
+    }
+}
+
 Plan *get_plan(const PlanRequest &req) {
   Runtime &r = rt();
   for (auto &p : r.plans)
@@ -631,7 +647,11 @@ Plan *get_plan(const PlanRequest &req) {
       return p.get();
     }
   if (OP_diags > 1) printf(" new execution plan #%zu for kernel %s\n", r.plans.size(), req.loop.c_str());
+  double t0, t1, cpu;
+  op_timers(&cpu, &t0);
   Plan *p = build(req);
+  op_timers(&cpu, &t1);
+  if (OP_diags > 1) printf(" plan built and verified in %.3f s\n", t1 - t0);
   p->count = 1;
   p->view.count = 1;
   r.plans.emplace_back(p);

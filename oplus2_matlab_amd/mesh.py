@@ -445,3 +445,65 @@ def jacobi_reference_problem(nn: int = 6) -> JacobiProblem:
                     a_l.append(0.25)
     return JacobiProblem(pp=np.array(pp_l, dtype=np.int32), A=np.array(a_l),
                          r=r, u=np.zeros(m * m), du=np.zeros(m * m), nn=nn)
+
+
+# ---------------------------------------------------------------------------
+# batches of meshes
+
+@dataclass
+class MeshBatch:
+    """Several airfoil meshes as one union mesh: node, cell, edge and bedge
+    numbers of mesh i are offset by the sizes of meshes 0..i-1, so one set of
+    each kind covers the batch.  cmesh / bmesh map cells / bedges to their
+    mesh; qinf holds each mesh's free stream (4 values per mesh)."""
+
+    union: AirfoilMesh
+    meshes: list
+    consts: list            # flow-constant arrays, one per mesh
+    cmesh: np.ndarray       # (ncell,) int32
+    bmesh: np.ndarray       # (nbedge,) int32
+    qinf: np.ndarray        # (nmesh, 4) float64
+    cell_off: np.ndarray    # (nmesh + 1,)
+
+    @property
+    def nmesh(self) -> int:
+        return len(self.meshes)
+
+
+def mesh_batch(nmesh: int = 64, seed: int = 1, cells_lo: int = 10_000,
+               cells_hi: int = 400_000, ordering: str = "cartesian") -> MeshBatch:
+    """nmesh O-meshes with cell counts log-uniform in [cells_lo, cells_hi]
+    (NI = 2 NJ), mach uniform in [0.3, 0.6] and alpha uniform in [0, 5]
+    degrees, all from `seed` (numpy PCG64)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    cells = np.exp(rng.uniform(np.log(cells_lo), np.log(cells_hi), nmesh))
+    mach = rng.uniform(0.3, 0.6, nmesh)
+    alpha = rng.uniform(0.0, 5.0, nmesh)
+    meshes, consts = [], []
+    for i in range(nmesh):
+        nj = max(2, int(round(math.sqrt(cells[i] / 2.0))))
+        meshes.append(o_mesh(2 * nj, nj, ordering, seed=seed * 1000 + i))
+        consts.append(flow_constants(mach=float(mach[i]), alpha_deg=float(alpha[i])).as_array())
+    return concat_meshes(meshes, consts)
+
+
+def concat_meshes(meshes: list, consts: list) -> MeshBatch:
+    nn = np.cumsum([0] + [m.nnode for m in meshes])
+    nc = np.cumsum([0] + [m.ncell for m in meshes])
+    ne = np.cumsum([0] + [m.nedge for m in meshes])
+    nb = np.cumsum([0] + [m.nbedge for m in meshes])
+    union = AirfoilMesh(
+        x=np.concatenate([m.x for m in meshes]),
+        cell=np.concatenate([m.cell + nn[i] for i, m in enumerate(meshes)]).astype(np.int32),
+        edge=np.concatenate([m.edge + nn[i] for i, m in enumerate(meshes)]).astype(np.int32),
+        ecell=np.concatenate([m.ecell + nc[i] for i, m in enumerate(meshes)]).astype(np.int32),
+        bedge=np.concatenate([m.bedge + nn[i] for i, m in enumerate(meshes)]).astype(np.int32),
+        becell=np.concatenate([m.becell + nc[i] for i, m in enumerate(meshes)]).astype(np.int32),
+        bound=np.concatenate([m.bound for m in meshes]).astype(np.int32),
+        ordering="batch")
+    cmesh = np.repeat(np.arange(len(meshes), dtype=np.int32), [m.ncell for m in meshes])
+    bmesh = np.repeat(np.arange(len(meshes), dtype=np.int32), [m.nbedge for m in meshes])
+    qinf = np.stack([c[6:10] for c in consts])
+    del ne, nb
+    return MeshBatch(union=union, meshes=meshes, consts=consts, cmesh=cmesh, bmesh=bmesh,
+                     qinf=qinf, cell_off=nc)

@@ -107,6 +107,9 @@ SIGNATURES = {
                                        C.c_int, C.POINTER(PlanSummary)]),
     "op2_plan_export": (C.c_int, [C.c_char_p, _P, C.c_int, C.POINTER(OpArg), C.c_int, C.c_int,
                                   _P, _P, _P, _P]),
+    "op2_autotune_loop": (C.c_int, [C.c_char_p, _P, C.c_int, C.POINTER(OpArg), C.c_int,
+                                    C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                    C.POINTER(C.c_double)]),
     "op2_set_deferred_reductions": (C.c_int, [C.c_int]),
     "op2_reduction_log": (C.c_int, [_P, C.c_int]),
     "op2_graph_begin": (C.c_int, []),
@@ -315,6 +318,17 @@ def sync() -> None:
 
 def stream() -> int:
     return _call("op2_stream")
+
+
+def autotune(name: str, set_: Set, args, reps: int = 5) -> dict:
+    """Times the loop under each candidate strategy and part size on the
+    live data (restored afterwards) and keeps the fastest; returns it."""
+    arr = (OpArg * len(args))(*args)
+    st, part, us = C.c_int(), C.c_int(), C.c_double()
+    if _call("op2_autotune_loop", _b(name), set_._p, len(args), arr, reps, C.byref(st),
+             C.byref(part), C.byref(us)) != 0:
+        raise OP2Error(lib().op2_last_error().decode())
+    return {"strategy": st.value, "part_size": part.value, "us": us.value}
 
 
 def set_deferred_reductions(on: bool) -> bool:

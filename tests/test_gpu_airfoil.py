@@ -272,3 +272,38 @@ def test_graph_capture_needs_prepared_loops(runtime):
         a.res_calc()
     op2.graph_end()
     op2.set_deferred_reductions(False)
+
+
+@pytest.mark.parametrize("strategy", [op2.STRATEGY_OWNER, op2.STRATEGY_OWNER_GATHER])
+@pytest.mark.parametrize("part", [32, 512, 1024])
+def test_owner_tile_sizes(runtime, strategy, part):
+    """Owner-compute tiles of every kernel shape (positions and owned cells
+    per thread) against the oracle."""
+    note(f"owner tiles fp64 48x24 random, strategy={strategy} part={part}")
+    configure(strategy, part)
+    mesh = M.o_mesh(48, 24, "random", seed=4)
+    c = M.flow_constants().as_array()
+    hist, q, _, _ = AF.run_airfoil(mesh, c, 10, "double")
+    st, oh = O.airfoil_seq(mesh, c, 10, "f64")
+    assert np.max(np.abs(hist - oh) / oh) < TOL64
+    assert relmax(q, st.q) < TOL64
+
+
+def test_autotuner_keeps_state_and_parity(runtime):
+    """The autotuner runs res_calc and bres_calc under every candidate on
+    the live mesh; the state must be unchanged afterwards and the tuned run
+    must still match the oracle."""
+    note("autotune res_calc/bres_calc on 48x24 random")
+    mesh = M.o_mesh(48, 24, "random", seed=6)
+    c = M.flow_constants().as_array()
+    a = AF.Airfoil(mesh, c, "double")
+    a.iterate()
+    before = [op2.op_fetch_data(d).copy() for d in (a.p_q, a.p_res, a.p_adt)]
+    tuned = a.autotune(reps=2)
+    note(f"tuned: {tuned}")
+    after = [op2.op_fetch_data(d).copy() for d in (a.p_q, a.p_res, a.p_adt)]
+    assert np.array_equal(before[0], after[0]) and np.array_equal(before[1], after[1])
+    assert tuned["res_calc"]["strategy"] in (1, 2, 4, 5)
+    hist = [a.iterate() for _ in range(9)]
+    st, oh = O.airfoil_seq(mesh, c, 10, "f64")
+    assert np.max(np.abs(np.array(hist) - oh[1:]) / oh[1:]) < TOL64
