@@ -18,6 +18,11 @@ double): its throughput, speed-up over FP64, and its error against the FP64
 run after --accuracy-iters iterations (rms history relative error and
 max |q - q64|).
 
+batched: 64 meshes of 10k-400k cells with their own mach and alpha
+(7.12M cells) advanced as one union mesh, each boundary edge reading its
+mesh's free stream and update summing each mesh's rms; the iteration is a
+hipGraph as above.  value in cell-updates/s and its ratio to the single mesh.
+
 cpu_baseline: the reference's OpenMP schedule (the oracle's bit-exact
 restatement of airfoil/<loop>_kernel.cpp on the reference plan) on this
 host's cores, over a bounded number of iterations of the same mesh.
@@ -53,6 +58,8 @@ def parse():
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-form measurement")
     ap.add_argument("--accuracy-iters", type=int, default=1000,
                     help="iterations of the FP32-vs-FP64 accuracy run (0 = skip)")
+    ap.add_argument("--batch-meshes", type=int, default=64,
+                    help="meshes of the batched measurement (0 = skip)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="target CPU time of the cpu_baseline sample (0 = skip)")
     return ap.parse_args()
@@ -99,6 +106,43 @@ def measure(mesh, consts, dtype, a):
         op2.op_exit()
     return {"value": mesh.ncell * a.steps / dt_graph, "ms_per_step": 1e3 * dt_graph / a.steps,
             "eager_ms_per_step": 1e3 * dt_eager / a.steps, "loops": loops, "rms_last": rms_last,
+            "tuned": tuned}
+
+
+def measure_batch(a):
+    """64 meshes (10k-400k cells, own mach and alpha) advanced together: one
+    graph-captured iteration of the union mesh, replayed; per-mesh rms in the
+    device log."""
+    from oplus2_matlab_amd import airfoil as AF
+    from oplus2_matlab_amd import mesh as M
+    from oplus2_matlab_amd import op2
+
+    batch = M.mesh_batch(nmesh=a.batch_meshes, seed=1)
+    op2.op_init([], 0)
+    try:
+        if a.strategy not in ("auto", "tune") or a.part:
+            op2.set_loop_config("res_calc", max(STRATEGIES[a.strategy], 0), a.part, 0)
+        af = AF.AirfoilBatch(batch, "double")
+        tuned = af.autotune() if a.strategy == "tune" else None
+        for _ in range(a.warmup):
+            af.iterate()
+        gid = af.capture_iteration()
+        af.run_graph(gid, a.warmup)
+        op2.sync()
+        t0 = time.perf_counter()
+        hist = af.run_graph(gid, a.steps)
+        op2.sync()
+        dt = time.perf_counter() - t0
+        op2.set_deferred_reductions(False)
+    finally:
+        op2.op_exit()
+    ncell = batch.union.ncell
+    return {"value": ncell * a.steps / dt, "unit": "cell-updates/s",
+            "ms_per_step": 1e3 * dt / a.steps, "meshes": batch.nmesh, "cells": ncell,
+            "edges": batch.union.nedge,
+            "cells_min": int(min(m.ncell for m in batch.meshes)),
+            "cells_max": int(max(m.ncell for m in batch.meshes)),
+            "rms_last_min": float(hist[-1].min()), "rms_last_max": float(hist[-1].max()),
             "tuned": tuned}
 
 
@@ -200,6 +244,10 @@ def main():
                                "loops": r32["loops"]}
         if a.accuracy_iters > 0:
             result["fp32_form"]["accuracy"] = accuracy(mesh, consts, a.accuracy_iters)
+    if a.batch_meshes > 0:
+        rb = measure_batch(a)
+        rb["ratio_vs_single_mesh"] = rb["value"] / r64["value"]
+        result["batched"] = rb
     result["cpu_baseline"] = cpu_baseline(mesh, consts, a.cpu_seconds) if a.cpu_seconds > 0 else None
     print(json.dumps(result))
 

@@ -187,6 +187,106 @@ def run_airfoil(mesh, consts, niter: int, dtype: str = "double", graph: bool = F
     return hist, q.copy(), adt.copy(), res.copy()
 
 
+class AirfoilBatch(Airfoil):
+    """A batch of airfoil meshes (mesh.MeshBatch) advanced together as one
+    union mesh.  The loops are the airfoil loops with two extra arguments:
+    bres_calc reads its boundary edge's mesh's free stream through
+    bedge -> mesh (a dat on the meshes set instead of the qinf constant) and
+    update increments each mesh's own rms through cell -> mesh.  Every mesh
+    then marches exactly as if it ran alone (no edge crosses meshes).
+
+    The per-mesh rms sums come from the runtime's reduction log (deferred
+    reductions are switched on): each update call logs one sum per mesh, in
+    mesh order."""
+
+    def __init__(self, batch, dtype: str = "double"):
+        c0 = np.asarray(batch.consts[0], dtype=np.float64)
+        for c in batch.consts:
+            if not np.array_equal(np.asarray(c)[:4], c0[:4]):
+                raise ValueError("meshes of a batch share gam, gm1, cfl and eps")
+        real = np.float64 if dtype == "double" else np.float32
+        super().__init__(batch.union, c0, dtype, q0=batch.qinf[batch.cmesh].astype(real))
+        self.batch = batch
+        self.nmesh = batch.nmesh
+        self.mesh_cells = np.diff(batch.cell_off).astype(np.float64)
+        self.meshes = op_decl_set(batch.nmesh, "meshes")
+        self.pcmesh = op_decl_map(self.cells, self.meshes, 1, batch.cmesh, "pcmesh")
+        self.pbmesh = op_decl_map(self.bedges, self.meshes, 1, batch.bmesh, "pbmesh")
+        self.qinf = np.ascontiguousarray(batch.qinf.astype(real).reshape(-1))
+        self.p_qinf = op_decl_dat(self.meshes, 4, dtype, self.qinf, "p_qinf")
+        self.rms_m = np.zeros(batch.nmesh, dtype=np.float64)
+        self.p_rms = op_decl_dat(self.meshes, 1, "double", self.rms_m, "p_rms")
+        op2.set_deferred_reductions(True)
+
+    def bres_calc_args(self):
+        return super().bres_calc_args() + (
+            op_arg_dat(self.p_qinf, 0, self.pbmesh, 4, self.t, OP_READ),)
+
+    def update(self):
+        t = self.t
+        op_par_loop(op2.update, "update", self.cells,
+                    op_arg_dat(self.p_qold, -1, OP_ID, 4, t, OP_READ),
+                    op_arg_dat(self.p_q, -1, OP_ID, 4, t, OP_WRITE),
+                    op_arg_dat(self.p_res, -1, OP_ID, 4, t, OP_RW),
+                    op_arg_dat(self.p_adt, -1, OP_ID, 1, t, OP_READ),
+                    op_arg_dat(self.p_rms, 0, self.pcmesh, 1, "double", OP_INC))
+
+    def _history(self, log: np.ndarray, n: int) -> np.ndarray:
+        if log.size != 2 * self.nmesh * n:
+            raise op2.OP2Error(f"reduction log holds {log.size} entries, "
+                               f"expected {2 * self.nmesh * n}")
+        second = log.reshape(n, 2, self.nmesh)[:, 1, :]
+        return np.sqrt(second / self.mesh_cells)
+
+    def iterate(self) -> np.ndarray:
+        """One iteration of every mesh; returns the nmesh values of
+        sqrt(rms / ncell) of the second stage."""
+        op2.reduction_log()
+        self.save_soln()
+        for _ in range(2):
+            self.adt_calc()
+            self.res_calc()
+            self.bres_calc()
+            self.update()
+        return self._history(op2.reduction_log(), 1)[0]
+
+    def run_graph(self, gid: int, niter: int, chunk: int = 0) -> np.ndarray:
+        """Replays the captured iteration niter times; returns the
+        (niter, nmesh) rms history."""
+        per_chunk = max(1, op2.REDUCTION_LOG_CAPACITY // (2 * self.nmesh))
+        chunk = min(chunk, per_chunk) if chunk > 0 else per_chunk
+        op2.reduction_log()
+        hist = []
+        done = 0
+        while done < niter:
+            n = min(chunk, niter - done)
+            op2.graph_launch(gid, n)
+            hist.append(self._history(op2.reduction_log(), n))
+            done += n
+        return np.concatenate(hist) if hist else np.zeros((0, self.nmesh))
+
+    def mesh_slice(self, arr: np.ndarray, i: int) -> np.ndarray:
+        """Rows of mesh i in a per-cell array of the union."""
+        off = self.batch.cell_off
+        return arr[off[i]:off[i + 1]]
+
+
+def run_airfoil_batch(batch, niter: int, dtype: str = "double", graph: bool = False):
+    """Runs `niter` iterations of every mesh of the batch; returns the
+    (niter, nmesh) rms history and the union's q after fetching.  The caller
+    owns op_init/op_exit."""
+    a = AirfoilBatch(batch, dtype)
+    if not graph or niter < 2:
+        hist = np.stack([a.iterate() for _ in range(niter)]) if niter else np.zeros((0, a.nmesh))
+    else:
+        first = a.iterate()
+        gid = a.capture_iteration()
+        hist = np.concatenate([first[None, :], a.run_graph(gid, niter - 1)])
+    op2.set_deferred_reductions(False)
+    q, _, _, _ = a.fetch()
+    return hist, q.copy()
+
+
 class Jacobi:
     """The jac1 program's sets, map and dats (jac1/jac.cpp:128-139)."""
 

@@ -220,6 +220,7 @@ static void clear_all() {
   r.deferred = false;
   for (auto &p : r.plans) p->release();
   r.plans.clear();
+  release_exec_caches();
   for (int i = 0; i < OP_dat_index; i++) {
     dev_free(OP_dat_list[i]->data_d);
     free(OP_dat_list[i]);

@@ -1064,8 +1064,13 @@ struct SegPlan {
 
 constexpr int kSegUpc = 4;  // cells per thread in k_update_seg
 
-SegPlan &seg_plan(op_map m) {
+std::vector<SegPlan> &seg_cache() {
   static std::vector<SegPlan> cache;
+  return cache;
+}
+
+SegPlan &seg_plan(op_map m) {
+  std::vector<SegPlan> &cache = seg_cache();
   for (auto &sp : cache)
     if (sp.map == m) return sp;
   const int n = m->from->size, nseg = m->to->size;
@@ -1092,6 +1097,20 @@ SegPlan &seg_plan(op_map m) {
   cache.push_back(sp);
   return cache.back();
 }
+
+}  // namespace
+
+// Frees the cached segment plans (their maps are about to be freed).
+void release_exec_caches() {
+  for (auto &sp : seg_cache()) {
+    dev_free(sp.d_seg_off);
+    dev_free(sp.d_pslot);
+    dev_free(sp.d_partials);
+  }
+  seg_cache().clear();
+}
+
+namespace {
 
 template <typename T>
 void run_update_batch(op_set set, op_arg *a) {

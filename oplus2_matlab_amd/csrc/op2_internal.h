@@ -168,6 +168,7 @@ Runtime &rt();
 void ensure_device();
 void *dev_alloc(size_t bytes);
 void dev_free(void *p);
+void release_exec_caches();
 void h2d(void *dst, const void *src, size_t bytes);
 void d2h(void *dst, const void *src, size_t bytes);
 void ensure_map_device(op_map m);

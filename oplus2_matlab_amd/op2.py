@@ -336,7 +336,10 @@ def set_deferred_reductions(on: bool) -> bool:
     return bool(_call("op2_set_deferred_reductions", int(on)))
 
 
-def reduction_log(max_entries: int = 1 << 16) -> np.ndarray:
+REDUCTION_LOG_CAPACITY = 1 << 16  # results the device log holds between reads
+
+
+def reduction_log(max_entries: int = REDUCTION_LOG_CAPACITY) -> np.ndarray:
     """Pending deferred-reduction results, oldest first (waits for the GPU)."""
     out = np.zeros(max_entries, np.float64)
     n = _call("op2_reduction_log", out.ctypes.data, max_entries)

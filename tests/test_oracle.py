@@ -239,3 +239,22 @@ def test_mixed_form_close_to_f64():
     _, h64 = O.airfoil_seq(mesh, c, 50, "f64")
     _, hmx = O.airfoil_seq(mesh, c, 50, "mix")
     assert np.abs(hmx - h64).max() / np.abs(h64).max() < 1e-4
+
+
+def test_mesh_batch_union_is_disjoint_meshes():
+    """The union mesh of a batch marches each of its meshes exactly as alone
+    (same free stream for all, so the plain airfoil loops apply): a wrong
+    node, cell or edge offset in concat_meshes couples the meshes."""
+    meshes = [M.o_mesh(16, 8, "random", seed=s) for s in (1, 2, 3)]
+    c = M.flow_constants().as_array()
+    batch = M.concat_meshes(meshes, [c] * 3)
+    assert np.all(np.diff(batch.cmesh) >= 0) and np.all(np.diff(batch.bmesh) >= 0)
+    assert batch.cell_off[-1] == batch.union.ncell
+    st, h = O.airfoil_seq(batch.union, c, 5, "f64")
+    sums = np.zeros(5)
+    for i, m in enumerate(meshes):
+        sti, hi = O.airfoil_seq(m, c, 5, "f64")
+        q = st.q.reshape(-1, 4)[batch.cell_off[i]:batch.cell_off[i + 1]]
+        assert np.array_equal(q, sti.q.reshape(-1, 4))
+        sums += hi ** 2 * m.ncell
+    np.testing.assert_allclose(h, np.sqrt(sums / batch.union.ncell), rtol=1e-13)
