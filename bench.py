@@ -42,7 +42,7 @@ sys.path.insert(0, ROOT)
 METRIC = "airfoil cell-updates/s & res_calc HBM GB/s (FP64 vs FP32 form) on 1 MI355X"
 HBM_PEAK_GBS = 8000.0
 STRATEGIES = {"auto": 0, "coloured": 1, "atomic": 2, "direct": 3, "owner": 4, "owner-gather": 5,
-              "tune": -1}
+              "tune": -1, "tune-offline": -1}
 
 
 def parse():
@@ -53,8 +53,11 @@ def parse():
     ap.add_argument("--mesh", default="2400x1200", help="NIxNJ O-mesh cells")
     ap.add_argument("--ordering", default="cartesian", choices=["cartesian", "random", "rcm"])
     ap.add_argument("--strategy", default="tune", choices=list(STRATEGIES),
-                    help="res_calc strategy; 'tune' runs the runtime's autotuner first")
+                    help="res_calc strategy; 'tune' runs the runtime's online autotuner in "
+                         "the warmup iterations, 'tune-offline' its offline one")
     ap.add_argument("--part", type=int, default=0, help="res_calc part size (0 = default)")
+    ap.add_argument("--renumber", choices=["on", "off"], default="on",
+                    help="renumber the sets for locality inside the runtime (op2_renumber)")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-form measurement")
     ap.add_argument("--accuracy-iters", type=int, default=1000,
                     help="iterations of the FP32-vs-FP64 accuracy run (0 = skip)")
@@ -72,10 +75,11 @@ def measure(mesh, consts, dtype, a):
 
     op2.op_init([], 0)
     try:
-        if a.strategy not in ("auto", "tune") or a.part:
+        if STRATEGIES[a.strategy] >= 0 and (a.strategy != "auto" or a.part):
             op2.set_loop_config("res_calc", max(STRATEGIES[a.strategy], 0), a.part, 0)
-        af = AF.Airfoil(mesh, consts, dtype)
-        tuned = af.autotune() if a.strategy == "tune" else None
+        af = AF.Airfoil(mesh, consts, dtype, renumber=a.renumber == "on")
+        tuned = (af.autotune(reps=3, online=True) if a.strategy == "tune" else
+                 af.autotune() if a.strategy == "tune-offline" else None)
         for _ in range(a.warmup):
             af.iterate()
         op2.sync()
@@ -120,10 +124,11 @@ def measure_batch(a):
     batch = M.mesh_batch(nmesh=a.batch_meshes, seed=1)
     op2.op_init([], 0)
     try:
-        if a.strategy not in ("auto", "tune") or a.part:
+        if STRATEGIES[a.strategy] >= 0 and (a.strategy != "auto" or a.part):
             op2.set_loop_config("res_calc", max(STRATEGIES[a.strategy], 0), a.part, 0)
-        af = AF.AirfoilBatch(batch, "double")
-        tuned = af.autotune() if a.strategy == "tune" else None
+        af = AF.AirfoilBatch(batch, "double", renumber=a.renumber == "on")
+        tuned = (af.autotune(reps=3, online=True) if a.strategy == "tune" else
+                 af.autotune() if a.strategy == "tune-offline" else None)
         for _ in range(a.warmup):
             af.iterate()
         gid = af.capture_iteration()
@@ -223,6 +228,7 @@ def main():
                                f"replayed as a hipGraph",
                    "cells": mesh.ncell, "edges": mesh.nedge, "ordering": a.ordering,
                    "strategy": a.strategy, "part_size": a.part or None,
+                   "renumbered": a.renumber == "on",
                    "tuned": r64["tuned"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,

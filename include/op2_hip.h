@@ -163,6 +163,39 @@ int op2_set_loop_config(char const *name, int strategy, int part_size,
 int op2_autotune_loop(char const *name, op_set set, int nargs, op_arg *args, int reps,
                       int *best_strategy, int *best_part, double *best_us);
 
+/* Online autotuner: from now on, the calls of loop `name` run each
+ * candidate strategy and part size in turn, one untimed call then `reps`
+ * timed calls per candidate, and every call is a real execution of the loop
+ * (so the candidates are timed inside the program's own loop sequence, with
+ * its cache state).  After the last candidate the fastest becomes the loop's
+ * configuration and the losers' plans are freed.  Loops that cannot be tuned
+ * (direct loops, loops with OP_GBL results) are refused at their first call.
+ * Graph capture is refused while a tuning is unfinished.  Returns 0. */
+int op2_autotune_online(char const *name, int reps);
+
+/* Result of an online tuning: 1 when finished (outputs set), 0 while it
+ * runs, -1 if none was started for `name`. */
+int op2_autotune_online_result(char const *name, int *best_strategy, int *best_part,
+                               double *best_us);
+
+/* Renumbering for locality: the op_partition planned by the reference
+ * (doc/user.tex:351-356, doc/dev.tex:191-206), done inside the runtime.
+ * The target set of `base` (cells, for the edge -> cell map) is ordered by
+ * recursive bisection of its adjacency graph (aligned runs of up to 512
+ * elements are compact pieces); every set connected to it through the
+ * declared maps follows (elements sorted by the first element of an ordered
+ * set they reference, or numbered by first touch).  Device data and maps
+ * are permuted; the user's arrays keep the user's numbering, op_fetch_data
+ * returns data in it, and dats and maps declared later are translated.
+ * Plans (and op_plan_get views) use the internal numbering.  Call before
+ * capturing graphs; at most once per op_init.  Returns 0. */
+int op2_renumber(op_map base);
+
+/* The internal number of every element of `set` (new_of_old[set size]; may
+ * be NULL): returns 1 if the set was renumbered, 0 if its numbering is the
+ * user's. */
+int op2_renumber_export(op_set set, int *new_of_old);
+
 /* Device seconds, call count and algorithmic bytes per call of a loop
  * (device time from HIP events on the runtime stream).  Returns 0 if the
  * loop has run. */
@@ -199,17 +232,20 @@ int op2_graph_destroy(int id);
 /* Device pointer of a dat (for zero-copy interop). */
 void *op2_dat_device_ptr(op_dat dat);
 
-/* Plan introspection for the plan of the given strategy (COLOURED, ATOMIC
- * or OWNER) and part size (0 = 512; targets per tile for OWNER): block
- * count, block colours, largest block, largest
+/* Plan introspection for the plan of the given strategy (COLOURED, ATOMIC,
+ * OWNER or OWNER_GATHER) and part size (0 = 512; targets per tile for the
+ * owner strategies): block count, block colours, largest block, largest
  * number of staged elements of one block (summed over its maps), mean
- * thread colours, and the number of violations found by the exact plan
+ * thread colours, the number of violations found by the exact plan
  * verifier (0 = every block and thread colouring is valid and every local
- * index maps back to the user's map). */
+ * index maps back to the user's map), and for owner plans the largest
+ * number of increments of one target (up to 4 they are packed in one
+ * 8-byte record per target). */
 typedef struct {
   int nblocks, ncolors, max_block, max_staged, strategy, part_size;
   double mean_thread_colours;
   long long verify_errors;
+  int max_incs;
 } op2_plan_summary;
 int op2_plan_summary_get(char const *name, op_set set, int nargs, op_arg *args, int strategy,
                          int part_size, op2_plan_summary *out);

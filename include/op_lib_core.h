@@ -40,13 +40,14 @@ typedef struct {
 } op_set_core;
 typedef op_set_core *op_set;
 
-/* op_lib_core.h:83-92; map_d is the device copy of map */
+/* op_lib_core.h:83-92; map_d is the device copy of map, map_i the
+ * runtime's renumbered host copy (NULL unless the sets were renumbered) */
 typedef struct {
   int index;
   op_set from, to;
   int dim, *map;
   char const *name;
-  int *map_d;
+  int *map_d, *map_i;
 } op_map_core;
 typedef op_map_core *op_map;
 

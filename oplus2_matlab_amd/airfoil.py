@@ -19,7 +19,8 @@ class Airfoil:
     """Sets, maps and dats of one airfoil mesh, declared as
     airfoil/airfoil.cpp:165-189 declares them."""
 
-    def __init__(self, mesh, consts: np.ndarray, dtype: str = "double", q0=None, res0=None):
+    def __init__(self, mesh, consts: np.ndarray, dtype: str = "double", q0=None, res0=None,
+                 renumber: bool = False):
         self.real = np.float64 if dtype == "double" else np.float32
         self.t = dtype
         self.mesh = mesh
@@ -56,6 +57,10 @@ class Airfoil:
             op_decl_const2(n, 1, dtype, np.array([consts[i]], dtype=real))
         op_decl_const2("qinf", 4, dtype, np.asarray(consts[6:10], dtype=real))
         self.rms = np.zeros(1, dtype=np.float64)
+        if renumber:
+            # the runtime reorders cells (and through the maps edges, nodes,
+            # bedges) for locality; fetched data keeps the mesh's numbering
+            op2.renumber(self.pecell)
 
     # the five loops, with the arguments of airfoil/airfoil.cpp:201-248
     def save_soln(self):
@@ -94,12 +99,27 @@ class Airfoil:
                 op_arg_dat(self.p_res, 0, self.pbecell, 4, t, OP_INC),
                 op_arg_dat(self.p_bound, -1, OP_ID, 1, "int", OP_READ))
 
-    def autotune(self, reps: int = 5) -> dict:
+    def autotune(self, reps: int = 5, online: bool = False) -> dict:
         """Picks strategy and part size of res_calc and bres_calc on this
-        mesh with the runtime's autotuner (state unchanged)."""
-        self.adt_calc()
-        return {"res_calc": op2.autotune("res_calc", self.edges, self.res_calc_args(), reps),
-                "bres_calc": op2.autotune("bres_calc", self.bedges, self.bres_calc_args(), reps)}
+        mesh.  Offline: the runtime times each loop alone on the live data
+        (state unchanged).  Online: the loops are timed inside real
+        iterations, which advance the solution; returns the choices and the
+        number of iterations that took."""
+        if not online:
+            self.adt_calc()
+            return {"res_calc": op2.autotune("res_calc", self.edges, self.res_calc_args(), reps),
+                    "bres_calc": op2.autotune("bres_calc", self.bedges, self.bres_calc_args(),
+                                              reps)}
+        names = ("res_calc", "bres_calc")
+        for n in names:
+            op2.autotune_online(n, reps)
+        for it in range(1, 1000):
+            self.iterate()
+            got = {n: op2.autotune_online_result(n) for n in names}
+            if all(got.values()):
+                got["iterations"] = it
+                return got
+        raise op2.OP2Error("online autotuning did not finish")
 
     def res_calc(self):
         op_par_loop(op2.res_calc, "res_calc", self.edges, *self.res_calc_args())
@@ -170,12 +190,13 @@ class Airfoil:
         return self.q.reshape(-1, 4), self.qold.reshape(-1, 4), self.adt, self.res.reshape(-1, 4)
 
 
-def run_airfoil(mesh, consts, niter: int, dtype: str = "double", graph: bool = False):
+def run_airfoil(mesh, consts, niter: int, dtype: str = "double", graph: bool = False,
+                renumber: bool = False):
     """Declares the mesh, runs `niter` iterations, returns (rms history,
     q, adt, res) after fetching the dats.  With graph=True the first
     iteration runs eagerly and the rest replay a captured hipGraph with
     deferred reductions.  The caller owns op_init/op_exit."""
-    a = Airfoil(mesh, consts, dtype)
+    a = Airfoil(mesh, consts, dtype, renumber=renumber)
     if not graph or niter < 2:
         hist = np.array([a.iterate() for _ in range(niter)])
     else:
@@ -199,13 +220,14 @@ class AirfoilBatch(Airfoil):
     reductions are switched on): each update call logs one sum per mesh, in
     mesh order."""
 
-    def __init__(self, batch, dtype: str = "double"):
+    def __init__(self, batch, dtype: str = "double", renumber: bool = False):
         c0 = np.asarray(batch.consts[0], dtype=np.float64)
         for c in batch.consts:
             if not np.array_equal(np.asarray(c)[:4], c0[:4]):
                 raise ValueError("meshes of a batch share gam, gm1, cfl and eps")
         real = np.float64 if dtype == "double" else np.float32
-        super().__init__(batch.union, c0, dtype, q0=batch.qinf[batch.cmesh].astype(real))
+        super().__init__(batch.union, c0, dtype, q0=batch.qinf[batch.cmesh].astype(real),
+                         renumber=renumber)
         self.batch = batch
         self.nmesh = batch.nmesh
         self.mesh_cells = np.diff(batch.cell_off).astype(np.float64)
@@ -271,11 +293,12 @@ class AirfoilBatch(Airfoil):
         return arr[off[i]:off[i + 1]]
 
 
-def run_airfoil_batch(batch, niter: int, dtype: str = "double", graph: bool = False):
+def run_airfoil_batch(batch, niter: int, dtype: str = "double", graph: bool = False,
+                      renumber: bool = False):
     """Runs `niter` iterations of every mesh of the batch; returns the
     (niter, nmesh) rms history and the union's q after fetching.  The caller
     owns op_init/op_exit."""
-    a = AirfoilBatch(batch, dtype)
+    a = AirfoilBatch(batch, dtype, renumber=renumber)
     if not graph or niter < 2:
         hist = np.stack([a.iterate() for _ in range(niter)]) if niter else np.zeros((0, a.nmesh))
     else:
@@ -290,7 +313,7 @@ def run_airfoil_batch(batch, niter: int, dtype: str = "double", graph: bool = Fa
 class Jacobi:
     """The jac1 program's sets, map and dats (jac1/jac.cpp:128-139)."""
 
-    def __init__(self, prob, alpha: float, dtype: str = "double"):
+    def __init__(self, prob, alpha: float, dtype: str = "double", renumber: bool = False):
         real = np.float64 if dtype == "double" else np.float32
         self.t = dtype
         self.real = real
@@ -307,6 +330,8 @@ class Jacobi:
         self.p_u = op_decl_dat(self.nodes, 1, dtype, self.u, "p_u")
         self.p_du = op_decl_dat(self.nodes, 1, dtype, self.du, "p_du")
         op_decl_const2("alpha", 1, dtype, np.array([alpha], dtype=real))
+        if renumber:
+            op2.renumber(self.ppedge)
 
     def iterate(self, beta: float):
         """One iteration of jac1/jac.cpp:147-163; returns (u_sum, u_max)."""
@@ -328,8 +353,9 @@ class Jacobi:
         return float(u_sum[0]), float(u_max[0])
 
 
-def run_jacobi(prob, niter: int, alpha: float, beta: float, dtype: str = "double"):
-    j = Jacobi(prob, alpha, dtype)
+def run_jacobi(prob, niter: int, alpha: float, beta: float, dtype: str = "double",
+               renumber: bool = False):
+    j = Jacobi(prob, alpha, dtype, renumber=renumber)
     hs, hm = [], []
     for _ in range(niter):
         s, m = j.iterate(beta)

@@ -97,7 +97,7 @@ void ensure_map_device(op_map m) {
   if (m->map_d) return;
   size_t bytes = sizeof(int) * (size_t)m->dim * m->from->size;
   m->map_d = (int *)dev_alloc(bytes);
-  h2d(m->map_d, m->map, bytes);
+  h2d(m->map_d, map_host(m), bytes);
 }
 
 void *reduct_scratch(size_t bytes) {
@@ -247,6 +247,16 @@ static void clear_all() {
     }
   r.timings.clear();
   r.consts.clear();
+  r.set_perm.clear();
+  r.set_segments.clear();
+  r.map_internal.clear();
+  for (auto &o : r.online)
+    for (auto &ev : o.second.events)
+      for (auto &e : ev) {
+        (void)hipEventDestroy(e.first);
+        (void)hipEventDestroy(e.second);
+      }
+  r.online.clear();
 }
 
 }  // namespace op2
@@ -347,7 +357,9 @@ op_map op_decl_map(op_set from, op_set to, int dim, int *imap, char const *name)
   m->map = imap;
   m->name = name;
   m->map_d = nullptr;  // copied to the device by the first loop that reads it
+  m->map_i = nullptr;
   OP_map_list[OP_map_index++] = m;
+  if (!rt().set_perm.empty()) renumber_map(m);
   return m;
   API_END(nullptr)
 }
@@ -379,7 +391,11 @@ op_dat op2_decl_dat_char(op_set set, int dim, char const *type, int size, char *
   if (!d) return nullptr;
   size_t bytes = (size_t)d->size * set->size;
   d->data_d = (char *)dev_alloc(bytes);
-  if (data)
+  if (data && set_new_of_old(set)) {
+    std::vector<char> internal(bytes);
+    permute_to_internal(d, data, internal.data());
+    h2d(d->data_d, internal.data(), bytes);
+  } else if (data)
     h2d(d->data_d, data, bytes);
   else
     OP2_HIP_CHECK(hipMemsetAsync(d->data_d, 0, bytes, rt().stream));
@@ -403,7 +419,14 @@ void op2_decl_const_char(int dim, char const *type, int size, char *data, char c
 void op2_fetch_data(op_dat dat) {
   API_BEGIN
   if (!dat || !dat->data_d || !dat->data) fail("op_fetch_data error -- dat has no device copy");
-  d2h(dat->data, dat->data_d, (size_t)dat->size * dat->set->size);
+  const size_t bytes = (size_t)dat->size * dat->set->size;
+  if (set_new_of_old(dat->set)) {
+    std::vector<char> internal(bytes);
+    d2h(internal.data(), dat->data_d, bytes);
+    permute_to_user(dat, internal.data(), dat->data);
+  } else {
+    d2h(dat->data, dat->data_d, bytes);
+  }
   API_END()
 }
 
@@ -655,6 +678,10 @@ int op2_graph_begin(void) {
   if (r.capturing) fail("op2: graph capture already active");
   if (!r.deferred)
     fail("op2: graph capture needs deferred reductions (op2_set_deferred_reductions(1))");
+  for (auto &o : r.online)
+    if (!o.second.done)
+      fail("op2: online autotuning of loop \"%s\" is unfinished; run the loop until it ends "
+           "before graph capture", o.first.c_str());
   timing_drain();
   reduct_scratch(256 + sizeof(double) * 2048 * 4);
   OP2_HIP_CHECK(hipStreamSynchronize(r.stream));
@@ -704,6 +731,48 @@ int op2_autotune_loop(char const *name, op_set set, int nargs, op_arg *args, int
   API_BEGIN
   if (!name || !set || nargs <= 0 || !args || reps <= 0) fail("op2_autotune_loop: invalid arguments");
   return autotune(name, set, nargs, args, reps, best_strategy, best_part, best_us);
+  API_END(-1)
+}
+
+int op2_autotune_online(char const *name, int reps) {
+  API_BEGIN
+  if (!name || reps <= 0) fail("op2_autotune_online: invalid arguments");
+  if (rt().capturing) fail("op2_autotune_online: not during graph capture");
+  OnlineTune t;
+  t.reps = reps;
+  rt().online[name] = t;
+  return 0;
+  API_END(-1)
+}
+
+int op2_autotune_online_result(char const *name, int *best_strategy, int *best_part,
+                               double *best_us) {
+  API_BEGIN
+  if (!name) fail("op2_autotune_online_result: invalid arguments");
+  auto it = rt().online.find(name);
+  if (it == rt().online.end()) return -1;
+  if (!it->second.done) return 0;
+  if (best_strategy) *best_strategy = it->second.best.strategy;
+  if (best_part) *best_part = it->second.best.part_size;
+  if (best_us) *best_us = it->second.best_us;
+  return 1;
+  API_END(-1)
+}
+
+int op2_renumber(op_map base) {
+  API_BEGIN
+  renumber(base);
+  return 0;
+  API_END(-1)
+}
+
+int op2_renumber_export(op_set set, int *new_of_old) {
+  API_BEGIN
+  if (!set) fail("op2_renumber_export: invalid set");
+  const std::vector<int> *p = set_new_of_old(set);
+  if (!p) return 0;
+  if (new_of_old) std::copy(p->begin(), p->end(), new_of_old);
+  return 1;
   API_END(-1)
 }
 

@@ -163,13 +163,14 @@ __global__ __launch_bounds__(NT) void k_res_calc(
 //  3. one thread per owned cell sums its increments in the plan's fixed
 //     order and adds them to res with a plain read-modify-write.
 // No atomics, no colours, one launch, deterministic.
-template <typename T, int OWN>
+template <typename T, int OWN, bool ELL>
 __global__ __launch_bounds__(NT) void k_res_calc_owner(
     const int *__restrict__ blk_off, int nblk, const int *__restrict__ node_off,
     const int *__restrict__ node_list, const int *__restrict__ cell_off,
     const int *__restrict__ cell_list, const ushort4 *__restrict__ loc,
     const int *__restrict__ owned_off, const int *__restrict__ owned,
-    const int *__restrict__ csr_off, const uint16_t *__restrict__ csr, const T *__restrict__ x,
+    const int *__restrict__ csr_off, const uint16_t *__restrict__ csr,
+    const ushort4 *__restrict__ ell, const T *__restrict__ x,
     const T *__restrict__ q, const T *__restrict__ adt, T *__restrict__ res,
     AirfoilConsts<T> k) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -188,10 +189,12 @@ __global__ __launch_bounds__(NT) void k_res_calc_owner(
   // overlaps the staging and the flux phases (OWN owned cells per thread)
   T rpre[OWN][4];
   int gown[OWN];
+  ushort4 e4[OWN];
 #pragma unroll
   for (int j = 0; j < OWN; j++) {
     const int i = tid + j * NT;
     gown[j] = i < no ? owned[o0 + i] : -1;
+    e4[j] = ELL && i < no ? ell[o0 + i] : make_ushort4(0xffff, 0xffff, 0xffff, 0xffff);
     if (gown[j] >= 0) Vec<T>::load4(res + 4 * (size_t)gown[j], rpre[j]);
   }
   for (int i = tid; i < nn; i += NT) {
@@ -223,9 +226,11 @@ __global__ __launch_bounds__(NT) void k_res_calc_owner(
     const int i = tid + j * NT;
     if (gown[j] < 0) continue;
     T acc[4] = {T(0), T(0), T(0), T(0)};
-    const int j1 = csr_off[o0 + i + 1];
-    for (int c = csr_off[o0 + i]; c < j1; c++) {
-      const unsigned e = csr[c];
+    const unsigned short es[4] = {e4[j].x, e4[j].y, e4[j].z, e4[j].w};
+    const int c0 = ELL ? 0 : csr_off[o0 + i], j1 = ELL ? 4 : csr_off[o0 + i + 1];
+    for (int c = c0; c < j1; c++) {
+      const unsigned e = ELL ? es[c] : csr[c];
+      if (ELL && e == 0xffffu) break;
       T f[4];
       Vec<T>::load4(fs + 4 * (e >> 2), f);
       if (e & 3u) {
@@ -242,34 +247,77 @@ __global__ __launch_bounds__(NT) void k_res_calc_owner(
   }
 }
 
+// Sum of the fluxes that increment one owned cell, in the plan's fixed
+// order: entries (position << 2 | inc slot) of its ELL record, slot 0 adds
+// the flux, slot 1 subtracts it (airfoil/res_calc.h:25-29).  fs is
+// component-major, fs[d * np + position].
+template <typename T>
+__device__ __forceinline__ void ell_sum(const T *fs, int np, ushort4 e4, T acc[4]) {
+  const unsigned short es[4] = {e4.x, e4.y, e4.z, e4.w};
+#pragma unroll
+  for (int s = 0; s < 4; s++) {
+    const unsigned e = es[s];
+    if (e == 0xffffu) break;
+    const int pos = e >> 2;
+    if (e & 3u) {
+#pragma unroll
+      for (int d = 0; d < 4; d++) acc[d] -= fs[d * np + pos];
+    } else {
+#pragma unroll
+      for (int d = 0; d < 4; d++) acc[d] += fs[d * np + pos];
+    }
+  }
+}
+
 // Owner-compute edge loop without staging (res_calc): the same tiles, but
 // each element position carries the global ids of its nodes and cells, and
 // x, q, adt are gathered straight from global memory (the tile's records
 // stay in the L1/L2).  Fluxes meet in LDS, component-major (8-byte lanes,
 // no bank conflicts), for the fixed-order sums of the owned cells.
-template <typename T, int OWN>
+// ELL: the owned cells' increment lists come as one 8-byte record each,
+// loaded at the start independently of everything else, instead of
+// csr_off/csr loads after the barrier.  OWNID: the tiles own consecutive
+// cells (renumbered sets), so no owned list is read.  The position ids and
+// ELL records are read once: non-temporal loads keep them from evicting
+// the gathered x, q and adt from the caches.
+typedef int op2_v4i __attribute__((ext_vector_type(4)));
+typedef unsigned short op2_v4us __attribute__((ext_vector_type(4)));
+
+template <typename T, int OWN, bool ELL, bool OWNID>
 __global__ __launch_bounds__(NT) void k_res_calc_owner_direct(
     const int *__restrict__ blk_off, int nblk, const int4 *__restrict__ ids,
     const int *__restrict__ owned_off, const int *__restrict__ owned,
-    const int *__restrict__ csr_off, const uint16_t *__restrict__ csr, const T *__restrict__ x,
-    const T *__restrict__ q, const T *__restrict__ adt, T *__restrict__ res,
-    AirfoilConsts<T> k) {
+    const int *__restrict__ csr_off, const uint16_t *__restrict__ csr,
+    const ushort4 *__restrict__ ell, const T *__restrict__ x, const T *__restrict__ q,
+    const T *__restrict__ adt, T *__restrict__ res, AirfoilConsts<T> k) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int b = xcd_block(blockIdx.x, nblk);
   const int p0 = blk_off[b], np = blk_off[b + 1] - p0;
   const int o0 = owned_off[b], no = owned_off[b + 1] - o0;
   T *fs = reinterpret_cast<T *>(smem);  // [4][np]
   const int tid = threadIdx.x;
-  T rpre[OWN][4];
   int gown[OWN];
+  ushort4 e4[OWN];
 #pragma unroll
   for (int j = 0; j < OWN; j++) {
     const int i = tid + j * NT;
-    gown[j] = i < no ? owned[o0 + i] : -1;
-    if (gown[j] >= 0) Vec<T>::load4(res + 4 * (size_t)gown[j], rpre[j]);
+    gown[j] = i < no ? (OWNID ? o0 + i : owned[o0 + i]) : -1;
+    if (ELL) {
+      if (i < no) {
+        const op2_v4us v = __builtin_nontemporal_load((const op2_v4us *)(ell + o0 + i));
+        e4[j] = make_ushort4(v.x, v.y, v.z, v.w);
+      } else {
+        e4[j] = make_ushort4(0xffff, 0xffff, 0xffff, 0xffff);
+      }
+    }
   }
+  T rpre[OWN][4];
+#pragma unroll
+  for (int j = 0; j < OWN; j++)
+    if (gown[j] >= 0) Vec<T>::load4(res + 4 * (size_t)gown[j], rpre[j]);
   for (int p = tid; p < np; p += NT) {
-    const int4 g = ids[p0 + p];
+    const op2_v4i gv = __builtin_nontemporal_load((const op2_v4i *)(ids + p0 + p));
+    const int4 g = make_int4(gv.x, gv.y, gv.z, gv.w);
     T x1[2], x2[2], q1[4], q2[4], f[4];
     Vec<T>::load2(x + 2 * (size_t)g.x, x1);
     Vec<T>::load2(x + 2 * (size_t)g.y, x2);
@@ -285,16 +333,20 @@ __global__ __launch_bounds__(NT) void k_res_calc_owner_direct(
     const int i = tid + j * NT;
     if (gown[j] < 0) continue;
     T acc[4] = {T(0), T(0), T(0), T(0)};
-    const int j1 = csr_off[o0 + i + 1];
-    for (int c = csr_off[o0 + i]; c < j1; c++) {
-      const unsigned e = csr[c];
-      const int pos = e >> 2;
-      if (e & 3u) {
+    if (ELL) {
+      ell_sum(fs, np, e4[j], acc);
+    } else {
+      const int j1 = csr_off[o0 + i + 1];
+      for (int c = csr_off[o0 + i]; c < j1; c++) {
+        const unsigned e = csr[c];
+        const int pos = e >> 2;
+        if (e & 3u) {
 #pragma unroll
-        for (int d = 0; d < 4; d++) acc[d] -= fs[d * np + pos];
-      } else {
+          for (int d = 0; d < 4; d++) acc[d] -= fs[d * np + pos];
+        } else {
 #pragma unroll
-        for (int d = 0; d < 4; d++) acc[d] += fs[d * np + pos];
+          for (int d = 0; d < 4; d++) acc[d] += fs[d * np + pos];
+        }
       }
     }
 #pragma unroll
@@ -621,10 +673,15 @@ __global__ __launch_bounds__(NT) void k_jac_res(
   const int n0 = node_off[b], nn = node_off[b + 1] - n0;
   T *us = reinterpret_cast<T *>(smem);
   T *ds = us + nn;
+  // nodes this block increments: the staged list also holds nodes it only
+  // reads (pp[2e+1]), which a block of the same colour may be incrementing,
+  // so only flagged nodes are written back
+  unsigned char *inc = reinterpret_cast<unsigned char *>(ds + nn);
   const int tid = threadIdx.x;
   for (int i = tid; i < nn; i += NT) {
     us[i] = u[node_list[n0 + i]];
     ds[i] = T(0);
+    inc[i] = 0;
   }
   __syncthreads();
   const int cb = COLOURED ? col_base[b] : 0;
@@ -636,6 +693,7 @@ __global__ __launch_bounds__(NT) void k_jac_res(
       const ushort2 L = loc[e0 + e];
       const int el = perm ? perm[e0 + e] : e0 + e;
       const T v = beta * A[el] * us[L.x];
+      inc[L.y] = 1;
       if (COLOURED)
         ds[L.y] += v;
       else
@@ -644,6 +702,7 @@ __global__ __launch_bounds__(NT) void k_jac_res(
     __syncthreads();
   }
   for (int i = tid; i < nn; i += NT) {
+    if (!inc[i]) continue;
     const int g = node_list[n0 + i];
     if (COLOURED)
       du[g] += ds[i];
@@ -929,13 +988,19 @@ void run_res_calc(const char *name, op_set set, int nargs, op_arg *a) {
     const int own = (p->max_owned + NT - 1) / NT;
     if (own > 4) fail("op2: owner tile of %s exceeds %d targets", name, 4 * NT);
     const size_t lds = owner_lds<T>(p, 2, 4, 4);
-    auto kern = own <= 1 ? k_res_calc_owner<T, 1>
-                         : own <= 2 ? k_res_calc_owner<T, 2> : k_res_calc_owner<T, 4>;
+    const bool ell = p->d_ell != nullptr;
+    auto kern = ell ? (own <= 1 ? k_res_calc_owner<T, 1, true>
+                                : own <= 2 ? k_res_calc_owner<T, 2, true>
+                                           : k_res_calc_owner<T, 4, true>)
+                    : (own <= 1 ? k_res_calc_owner<T, 1, false>
+                                : own <= 2 ? k_res_calc_owner<T, 2, false>
+                                           : k_res_calc_owner<T, 4, false>);
     allow_lds(kern, lds);
     hipLaunchKernelGGL(kern, dim3(p->nblocks), dim3(NT), lds, rt().stream,
                        p->d_blk_off, p->nblocks, nodes.d_off, nodes.d_list, cells.d_off,
                        cells.d_list, (const ushort4 *)p->d_loc, p->d_owned_off, p->d_owned,
-                       p->d_csr_off, p->d_csr, (const T *)a[0].dat->data_d,
+                       p->d_csr_off, p->d_csr, (const ushort4 *)p->d_ell,
+                       (const T *)a[0].dat->data_d,
                        (const T *)a[2].dat->data_d, (const T *)a[4].dat->data_d,
                        (T *)a[6].dat->data_d, airfoil_consts<T>());
     return;
@@ -945,12 +1010,21 @@ void run_res_calc(const char *name, op_set set, int nargs, op_arg *a) {
     const size_t lds = sizeof(T) * 4 * (size_t)p->max_positions;
     const int own = (p->max_owned + NT - 1) / NT;
     if (own > 4) fail("op2: owner tile of %s exceeds %d targets", name, 4 * NT);
-    auto kern = own <= 1 ? k_res_calc_owner_direct<T, 1>
-                         : own <= 2 ? k_res_calc_owner_direct<T, 2> : k_res_calc_owner_direct<T, 4>;
+    const bool ell = p->d_ell != nullptr;
+    const bool oid = p->owned_identity;
+    decltype(&k_res_calc_owner_direct<T, 1, true, true>) kern;
+#define OP2_G(O, E) \
+  (oid ? k_res_calc_owner_direct<T, O, E, true> : k_res_calc_owner_direct<T, O, E, false>)
+    if (ell)
+      kern = own <= 1 ? OP2_G(1, true) : own <= 2 ? OP2_G(2, true) : OP2_G(4, true);
+    else
+      kern = own <= 1 ? OP2_G(1, false) : own <= 2 ? OP2_G(2, false) : OP2_G(4, false);
+#undef OP2_G
     allow_lds(kern, lds);
     hipLaunchKernelGGL(kern, dim3(p->nblocks), dim3(NT), lds, rt().stream, p->d_blk_off,
                        p->nblocks, (const int4 *)p->d_ids, p->d_owned_off, p->d_owned,
-                       p->d_csr_off, p->d_csr, (const T *)a[0].dat->data_d,
+                       p->d_csr_off, p->d_csr, (const ushort4 *)p->d_ell,
+                       (const T *)a[0].dat->data_d,
                        (const T *)a[2].dat->data_d, (const T *)a[4].dat->data_d,
                        (T *)a[6].dat->data_d, airfoil_consts<T>());
     return;
@@ -1074,11 +1148,12 @@ SegPlan &seg_plan(op_map m) {
   for (auto &sp : cache)
     if (sp.map == m) return sp;
   const int n = m->from->size, nseg = m->to->size;
+  const int *mm = map_host(m);
   std::vector<int> off(nseg + 1, 0);
   for (int c = 0; c < n; c++) {
-    if (c > 0 && m->map[c] < m->map[c - 1])
+    if (c > 0 && mm[c] < mm[c - 1])
       fail("op2: batched update needs the cell -> mesh map %s to be non-decreasing", m->name);
-    off[m->map[c] + 1]++;
+    off[mm[c] + 1]++;
   }
   for (int i = 0; i < nseg; i++) off[i + 1] += off[i];
   const int ch = NT * kSegUpc;
@@ -1147,7 +1222,7 @@ void run_jac_res(const char *name, op_set set, int nargs, op_arg *a) {
   Plan *p = plan_for(name, set, nargs, a, strategy, part_of(name, 1024));
   const bool col = strategy == OP2_STRATEGY_COLOURED;
   const PlanGroup &nodes = p->groups[0];
-  const size_t lds = sizeof(T) * 2 * (size_t)nodes.max_count;
+  const size_t lds = (sizeof(T) * 2 + 1) * (size_t)nodes.max_count;  // us, ds, inc flags
   auto kern = col ? k_jac_res<T, true> : k_jac_res<T, false>;
   if (lds > 64 * 1024)
     OP2_HIP_CHECK(hipFuncSetAttribute((const void *)kern,
@@ -1259,6 +1334,124 @@ void dispatch(Loop l, const char *name, op_set set, int nargs, op_arg *a) {
   }
 }
 
+// Candidate configurations of a tunable loop, in trial order.
+std::vector<LoopConfig> tune_candidates(Loop l) {
+  std::vector<LoopConfig> c;
+  if (l == Loop::kResCalc) {
+    for (int part : {128, 256, 512}) c.push_back({OP2_STRATEGY_OWNER_GATHER, part, 0});
+    for (int part : {128, 256}) c.push_back({OP2_STRATEGY_OWNER, part, 0});
+  } else {
+    for (int part : {128, 256, 512}) c.push_back({OP2_STRATEGY_OWNER, part, 0});
+  }
+  c.push_back({OP2_STRATEGY_ATOMIC, 256, 0});
+  for (int part : {128, 256}) c.push_back({OP2_STRATEGY_COLOURED, part, 0});
+  return c;
+}
+
+void dispatch_any(Loop l, int rb, const char *name, op_set set, int nargs, op_arg *a) {
+  if (rb == 8)
+    dispatch<double>(l, name, set, nargs, a);
+  else
+    dispatch<float>(l, name, set, nargs, a);
+}
+
+// One call of a loop under online tuning: runs the current candidate (the
+// first call of each is untimed: it builds the plan), times it with events
+// on the runtime stream, and after the last candidate keeps the fastest.
+// A candidate that cannot run (a plan limit) is skipped and the call runs
+// under the loop's previous configuration instead.
+void online_call(const std::string &name, OnlineTune &ot, Loop l, int rb, op_set set, int nargs,
+                 op_arg *a) {
+  Runtime &r = rt();
+  if (l != Loop::kResCalc && l != Loop::kBresCalc && l != Loop::kJacRes)
+    fail("op2: loop \"%s\" cannot be autotuned (only indirect loops without global results)",
+         name.c_str());
+  if (r.capturing)
+    fail("op2: online autotuning of loop \"%s\" during graph capture", name.c_str());
+  if (!ot.started) {
+    ot.started = true;
+    ot.cands = tune_candidates(l);
+    ot.us.assign(ot.cands.size(), 0.0);
+    ot.events.assign(ot.cands.size(), {});
+    auto it = r.loop_configs.find(name);
+    ot.best = it != r.loop_configs.end() ? it->second : LoopConfig{};
+  }
+  const int per = ot.reps + 1;
+  const int ncand = (int)ot.cands.size();
+  const int idx = ot.call / per, sub = ot.call % per;
+  const LoopConfig before = ot.best;
+  r.loop_configs[name] = ot.cands[idx];
+  const int keep_mode = r.error_mode;
+  bool ran = false;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (sub > 0) {
+    OP2_HIP_CHECK(hipEventCreate(&e0));
+    OP2_HIP_CHECK(hipEventCreate(&e1));
+    OP2_HIP_CHECK(hipEventRecord(e0, r.stream));
+  }
+  const std::string keep_err = r.last_error;
+  try {
+    r.error_mode = OP2_ERRORS_RETURN;
+    dispatch_any(l, rb, name.c_str(), set, nargs, a);
+    ran = true;
+  } catch (const op2::Error &) {
+    (void)hipGetLastError();
+    r.last_error = keep_err;  // a candidate that cannot run is not an error of the call
+  }
+  r.error_mode = keep_mode;
+  if (sub > 0) {
+    OP2_HIP_CHECK(hipEventRecord(e1, r.stream));
+    ot.events[idx].push_back({e0, e1});
+  }
+  if (!ran) {
+    // skip the candidate and run this call under the previous configuration
+    ot.us[idx] = 1e300;
+    ot.call = (idx + 1) * per;
+    r.loop_configs[name] = before;
+    if (!before.strategy) r.loop_configs.erase(name);
+    dispatch_any(l, rb, name.c_str(), set, nargs, a);
+  } else {
+    ot.call++;
+  }
+  if (ot.call < ncand * per) return;
+  // all candidates ran: pick the fastest
+  OP2_HIP_CHECK(hipStreamSynchronize(r.stream));
+  int bi = -1;
+  for (int i = 0; i < ncand; i++) {
+    double sum = 0;
+    for (auto &e : ot.events[i]) {
+      float ms = 0;
+      if (ot.us[i] < 1e300) OP2_HIP_CHECK(hipEventElapsedTime(&ms, e.first, e.second));
+      sum += ms;
+      OP2_HIP_CHECK(hipEventDestroy(e.first));
+      OP2_HIP_CHECK(hipEventDestroy(e.second));
+    }
+    ot.events[i].clear();
+    if (ot.us[i] < 1e300) ot.us[i] = 1e3 * sum / ot.reps;
+    if (OP_diags > 0)
+      printf(" online autotune %s: strategy %d part %d: %.1f us\n", name.c_str(),
+             ot.cands[i].strategy, ot.cands[i].part_size, ot.us[i]);
+    if (ot.us[i] < 1e300 && (bi < 0 || ot.us[i] < ot.us[bi])) bi = i;
+  }
+  ot.done = true;
+  if (bi < 0) {
+    r.loop_configs[name] = before;
+    if (!before.strategy) r.loop_configs.erase(name);
+    fail("op2: no candidate configuration of loop \"%s\" ran", name.c_str());
+  }
+  ot.best = ot.cands[bi];
+  ot.best_us = ot.us[bi];
+  r.loop_configs[name] = ot.best;
+  for (size_t i = 0; i < r.plans.size();) {
+    Plan *p = r.plans[i].get();
+    if (p->loop == name && p->set == set &&
+        !(p->strategy == ot.best.strategy && p->part_size == ot.best.part_size))
+      drop_plan(p);
+    else
+      i++;
+  }
+}
+
 }  // namespace
 
 int par_loop(const char *name, op_set set, int nargs, op_arg *args) {
@@ -1314,10 +1507,15 @@ int par_loop(const char *name, op_set set, int nargs, op_arg *args) {
     gbl_max(args[4], h[1]);
     return 0;
   }
-  if (rb == 8)
-    dispatch<double>(l, name, set, nargs, args);
-  else
-    dispatch<float>(l, name, set, nargs, args);
+  auto ot = rt().online.find(name);
+  if (ot != rt().online.end() && !ot->second.done) {
+    online_call(ot->first, ot->second, l, rb, set, nargs, args);
+  } else {
+    if (rb == 8)
+      dispatch<double>(l, name, set, nargs, args);
+    else
+      dispatch<float>(l, name, set, nargs, args);
+  }
   check_launch(name);
   timing_end(t, start);
   return 0;
@@ -1365,6 +1563,7 @@ int autotune(const char *name, op_set set, int nargs, op_arg *args, int reps, in
   OP2_HIP_CHECK(hipEventCreate(&e1));
   double best = 1e300;
   int bs = 0, bp = 0;
+  const std::string keep_err = r.last_error;
   auto trial = [&](int strategy, int part) {
     r.loop_configs[name] = LoopConfig{strategy, part, 0};
     const size_t nplans = r.plans.size();
@@ -1389,6 +1588,7 @@ int autotune(const char *name, op_set set, int nargs, op_arg *args, int reps, in
       us = 1e3 * ms / reps;
     } catch (const op2::Error &) {
       (void)hipGetLastError();
+      r.last_error = keep_err;
     }
     if (OP_diags > 0) printf(" autotune %s: strategy %d part %d: %.1f us\n", name, strategy, part, us);
     if (us < best) {

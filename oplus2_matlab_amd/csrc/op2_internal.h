@@ -100,7 +100,11 @@ struct Plan {
   std::vector<int> owned;          // global target ids
   std::vector<int> csr_off;        // [owned.size()+1]
   std::vector<uint16_t> csr;       // (position in block << 2) | inc slot index
-  int max_positions = 0, max_owned = 0;
+  // the same lists padded to 4 entries per owned target (0xffff = none), when
+  // no target has more than 4: one 8-byte record per owned target
+  std::vector<uint16_t> ell;
+  int max_positions = 0, max_owned = 0, max_incs = 0;
+  bool owned_identity = false;  // owned[i] == i: tiles own consecutive targets
 
   // device copies
   int *d_blk_off = nullptr, *d_perm = nullptr, *d_col_off = nullptr,
@@ -109,7 +113,7 @@ struct Plan {
   std::vector<int> ids;            // [positions * slot_stride] global targets (gather form)
   int *d_ids = nullptr;
   int *d_owned_off = nullptr, *d_owned = nullptr, *d_csr_off = nullptr;
-  uint16_t *d_csr = nullptr;
+  uint16_t *d_csr = nullptr, *d_ell = nullptr;
 
   // the op_plan view returned by op_plan_core / op_plan_get
   op_plan view{};
@@ -131,6 +135,18 @@ struct LoopTiming {
     hipEvent_t start, stop;
   };
   std::vector<Pending> pending;
+};
+
+// Online autotuning state of one loop (op2_autotune_online).
+struct OnlineTune {
+  int reps = 3;
+  bool started = false, done = false;
+  std::vector<LoopConfig> cands;
+  std::vector<double> us;  // mean device time per candidate (1e300: failed)
+  std::vector<std::vector<std::pair<hipEvent_t, hipEvent_t>>> events;
+  int call = 0;
+  LoopConfig best;
+  double best_us = 0;
 };
 
 struct Runtime {
@@ -160,6 +176,13 @@ struct Runtime {
   // graph capture of loop sequences on the runtime stream
   bool capturing = false;
   std::vector<hipGraphExec_t> graphs;
+  std::map<std::string, OnlineTune> online;
+  // renumbering (op2_renumber): new number of every element of a renumbered
+  // set, the starts of its independently ordered pieces, and the
+  // renumbered host copies of the maps (op_map::map_i points into them)
+  std::map<op_set, std::vector<int>> set_perm;
+  std::map<op_set, std::vector<int>> set_segments;
+  std::map<op_map, std::vector<int>> map_internal;
 };
 
 constexpr unsigned kLogCap = 1u << 16;  // power of two
@@ -172,6 +195,14 @@ void release_exec_caches();
 void h2d(void *dst, const void *src, size_t bytes);
 void d2h(void *dst, const void *src, size_t bytes);
 void ensure_map_device(op_map m);
+
+// renumbering (op2_renumber.cpp)
+inline const int *map_host(op_map m) { return m->map_i ? m->map_i : m->map; }
+const std::vector<int> *set_new_of_old(op_set s);
+void renumber(op_map base);
+void renumber_map(op_map m);  // (re)builds m's internal copy for the current numbering
+void permute_to_internal(op_dat d, const char *user, char *internal);
+void permute_to_user(op_dat d, const char *internal, char *user);
 
 const Const &get_const(const char *name);
 LoopConfig config_for(const std::string &loop);

@@ -34,7 +34,9 @@ static bool same_key(const Plan &p, const PlanRequest &r) {
   return true;
 }
 
-static inline int target(op_map m, int idx, int e) { return m->map[(size_t)e * m->dim + idx]; }
+static inline int target(op_map m, int idx, int e) {
+  return map_host(m)[(size_t)e * m->dim + idx];
+}
 
 void Plan::upload() {
   if (uploaded) return;
@@ -63,17 +65,19 @@ void Plan::upload() {
     d_owned = (int *)up(owned.data(), sizeof(int) * std::max<size_t>(owned.size(), 1));
     d_csr_off = (int *)up(csr_off.data(), sizeof(int) * csr_off.size());
     d_csr = (uint16_t *)up(csr.data(), sizeof(uint16_t) * std::max<size_t>(csr.size(), 1));
+    if (!ell.empty()) d_ell = (uint16_t *)up(ell.data(), sizeof(uint16_t) * ell.size());
   }
 }
 
 void Plan::release() {
   for (void *p : {(void *)d_blk_off, (void *)d_perm, (void *)d_col_off, (void *)d_col_base,
                   (void *)d_blk_ncol, (void *)d_blkmap, (void *)d_loc, (void *)d_owned_off,
-                  (void *)d_owned, (void *)d_csr_off, (void *)d_csr, (void *)d_ids})
+                  (void *)d_owned, (void *)d_csr_off, (void *)d_csr, (void *)d_ell,
+                  (void *)d_ids})
     dev_free(p);
   d_ids = nullptr;
   d_owned_off = d_owned = d_csr_off = nullptr;
-  d_csr = nullptr;
+  d_csr = d_ell = nullptr;
   for (auto &g : groups) {
     dev_free(g.d_off);
     dev_free(g.d_list);
@@ -140,7 +144,8 @@ static void build_owner_layout(Plan &P, const std::vector<int> &inc_slots, int t
     for (int k = 0; k < ni; k++) tile_of[tgt(e, k)] = -1;
 
   int ntiles = 0;
-  if (ni >= 2) {
+  auto segs = rt().set_segments.find(om->to);
+  if (ni >= 2 && segs == rt().set_segments.end()) {
     // adjacency between targets incremented by one element
     std::vector<int> deg(nt + 1, 0);
     for (int e = 0; e < n; e++)
@@ -171,6 +176,19 @@ static void build_owner_layout(Plan &P, const std::vector<int> &inc_slots, int t
         }
       }
       ntiles++;
+    }
+  } else if (ni >= 2) {
+    // renumbered targets: aligned runs of `tile` numbers inside each piece
+    // of the renumbering are compact (one subtree of its bisection)
+    const std::vector<int> &st = segs->second;
+    size_t sg = 0;
+    int base = 0, cur = -1;
+    for (int t = 0; t < nt; t++) {
+      while (sg < st.size() && st[sg] == t) base = st[sg++], cur = -1;
+      if ((t - base) % tile == 0) cur = -1;
+      if (tile_of[t] != -1) continue;
+      if (cur < 0) cur = ntiles++;
+      tile_of[t] = cur;
     }
   } else {
     int count = 0;
@@ -203,6 +221,9 @@ static void build_owner_layout(Plan &P, const std::vector<int> &inc_slots, int t
   }
   for (int b = 0; b < ntiles; b++)
     P.max_owned = std::max(P.max_owned, P.owned_off[b + 1] - P.owned_off[b]);
+  P.owned_identity = true;
+  for (size_t i = 0; i < P.owned.size() && P.owned_identity; i++)
+    P.owned_identity = P.owned[i] == (int)i;
 
   // elements of each tile: those incrementing one of its targets (ascending)
   std::vector<int> cnt(ntiles + 1, 0);
@@ -252,6 +273,14 @@ static void build_owner_layout(Plan &P, const std::vector<int> &inc_slots, int t
           if (tile_of[t] == b)
             P.csr[fill[P.owned_off[b] + local[t]]++] = (uint16_t)(((pos - cnt[b]) << 2) | k);
         }
+  }
+  // positions stay below 1 << 14, so no entry equals the 0xffff pad
+  for (size_t i = 0; i < P.owned.size(); i++)
+    P.max_incs = std::max(P.max_incs, P.csr_off[i + 1] - P.csr_off[i]);
+  if (P.max_incs <= 4) {
+    P.ell.assign(4 * P.owned.size(), 0xffff);
+    for (size_t i = 0; i < P.owned.size(); i++)
+      for (int j = P.csr_off[i]; j < P.csr_off[i + 1]; j++) P.ell[4 * i + j - P.csr_off[i]] = P.csr[j];
   }
 }
 
@@ -533,6 +562,18 @@ static long long verify_owner(const Plan &P) {
     err++;
   else
     for (size_t i = 0; i < seen.size(); i++) err += seen[i] != (long long)i;
+  // packed records: the same entries in the same order, then padding
+  if (!P.ell.empty()) {
+    if (P.ell.size() != 4 * P.owned.size()) return err + 1;
+    for (size_t i = 0; i < P.owned.size(); i++) {
+      const int c0 = P.csr_off[i], nc = P.csr_off[i + 1] - c0;
+      if (nc > 4) {
+        err++;
+        continue;
+      }
+      for (int s = 0; s < 4; s++) err += P.ell[4 * i + s] != (s < nc ? P.csr[c0 + s] : 0xffff);
+    }
+  }
   return err;
 }
 
@@ -706,6 +747,7 @@ int op2_plan_summary_get(char const *name, op_set set, int nargs, op_arg *args, 
     out->part_size = p->part_size;
     out->mean_thread_colours = p->mean_thread_colours;
     out->verify_errors = verify_plan(*p);
+    out->max_incs = p->max_incs;
     int staged = 0;
     for (auto &g : p->groups) staged += g.max_count;
     out->max_staged = staged;

@@ -45,7 +45,7 @@ class _MapCore(C.Structure):
     _fields_ = [("index", C.c_int), ("from_", C.POINTER(_SetCore)),
                 ("to", C.POINTER(_SetCore)), ("dim", C.c_int),
                 ("map", C.POINTER(C.c_int)), ("name", C.c_char_p),
-                ("map_d", C.c_void_p)]
+                ("map_d", C.c_void_p), ("map_i", C.c_void_p)]
 
 
 class _DatCore(C.Structure):
@@ -65,7 +65,8 @@ class OpArg(C.Structure):
 class PlanSummary(C.Structure):
     _fields_ = [("nblocks", C.c_int), ("ncolors", C.c_int), ("max_block", C.c_int),
                 ("max_staged", C.c_int), ("strategy", C.c_int), ("part_size", C.c_int),
-                ("mean_thread_colours", C.c_double), ("verify_errors", C.c_longlong)]
+                ("mean_thread_colours", C.c_double), ("verify_errors", C.c_longlong),
+                ("max_incs", C.c_int)]
 
 
 # every extern "C" entry point of include/op2_hip.h: name -> (restype, argtypes)
@@ -97,6 +98,11 @@ SIGNATURES = {
     "op2_last_error": (C.c_char_p, []),
     "op2_clear_error": (None, []),
     "op2_set_loop_config": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int]),
+    "op2_renumber": (C.c_int, [_P]),
+    "op2_renumber_export": (C.c_int, [_P, _P]),
+    "op2_autotune_online": (C.c_int, [C.c_char_p, C.c_int]),
+    "op2_autotune_online_result": (C.c_int, [C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                             C.POINTER(C.c_double)]),
     "op2_loop_stats": (C.c_int, [C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_double),
                                  C.POINTER(C.c_double)]),
     "op2_timing_reset": (None, []),
@@ -329,6 +335,38 @@ def autotune(name: str, set_: Set, args, reps: int = 5) -> dict:
              C.byref(part), C.byref(us)) != 0:
         raise OP2Error(lib().op2_last_error().decode())
     return {"strategy": st.value, "part_size": part.value, "us": us.value}
+
+
+def renumber(base: "Map") -> None:
+    """Renumbers the sets for locality from the target set of `base`
+    (internal: fetched data stays in the user's numbering)."""
+    if _call("op2_renumber", base._p) != 0:
+        raise OP2Error(lib().op2_last_error().decode())
+
+
+def renumber_export(set_: "Set"):
+    """Internal number of every element of the set, or None if the set
+    keeps the user's numbering."""
+    out = np.zeros(set_.size, np.int32)
+    rc = _call("op2_renumber_export", set_._p, out.ctypes.data)
+    return out if rc == 1 else None
+
+
+def autotune_online(name: str, reps: int = 3) -> None:
+    """The next calls of loop `name` try each candidate configuration in
+    turn (every call a real execution); the fastest is kept."""
+    if _call("op2_autotune_online", _b(name), reps) != 0:
+        raise OP2Error(lib().op2_last_error().decode())
+
+
+def autotune_online_result(name: str):
+    """The chosen configuration once the online tuning of `name` has
+    finished, else None."""
+    st, part, us = C.c_int(), C.c_int(), C.c_double()
+    rc = _call("op2_autotune_online_result", _b(name), C.byref(st), C.byref(part), C.byref(us))
+    if rc < 0:
+        raise OP2Error(f"no online tuning of loop {name}")
+    return {"strategy": st.value, "part_size": part.value, "us": us.value} if rc == 1 else None
 
 
 def set_deferred_reductions(on: bool) -> bool:

@@ -88,3 +88,14 @@ def test_loop_config_validation():
     with pytest.raises(op2.OP2Error):
         op2.set_loop_config("res_calc", 1, 0, 100)
     op2.set_loop_config(None, 0, 0, 0)
+
+
+def test_online_autotune_arguments():
+    op2.lib().op_init_core(0, None, 0)
+    with pytest.raises(op2.OP2Error, match="invalid"):
+        op2.autotune_online("res_calc", 0)
+    with pytest.raises(op2.OP2Error, match="no online tuning"):
+        op2.autotune_online_result("never_tuned")
+    op2.autotune_online("res_calc", 2)
+    assert op2.autotune_online_result("res_calc") is None
+    op2.op_exit()

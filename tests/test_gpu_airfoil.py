@@ -144,9 +144,12 @@ def test_airfoil_fp32_form(runtime):
 # Jacobi
 
 @pytest.mark.parametrize("strategy", STRATS)
-def test_jacobi_matches_reference(runtime, golden_dir, strategy):
-    note(f"jacobi fp64 64x64 random, 200 iterations, strategy={strategy}")
-    configure(strategy, 1024 if strategy != op2.STRATEGY_OWNER else 256)
+@pytest.mark.parametrize("part", [128, 1024])
+def test_jacobi_matches_reference(runtime, golden_dir, strategy, part):
+    """Small blocks share many read-only nodes with blocks of the same
+    colour: only the nodes a block increments may be written back."""
+    note(f"jacobi fp64 64x64 random, 200 iterations, strategy={strategy} part={part}")
+    configure(strategy, part if strategy != op2.STRATEGY_OWNER else 256)
     fx = np.load(os.path.join(golden_dir, "jacobi_64_random_s1_f64.npz"))
     prob = M.jacobi_grid(64, seed=1)
     hs, hm, u, du = AF.run_jacobi(prob, int(fx["niter"]), 0.25, -0.25, "double")
@@ -307,3 +310,68 @@ def test_autotuner_keeps_state_and_parity(runtime):
     hist = [a.iterate() for _ in range(9)]
     st, oh = O.airfoil_seq(mesh, c, 10, "f64")
     assert np.max(np.abs(np.array(hist) - oh[1:]) / oh[1:]) < TOL64
+
+
+@pytest.mark.parametrize("strategy", [op2.STRATEGY_OWNER, op2.STRATEGY_OWNER_GATHER])
+def test_owner_cells_with_more_than_four_increments(runtime, strategy):
+    """Edges listed twice give cells 5-8 increments, beyond the packed
+    4-entry records of the owner plans: those plans keep per-cell lists."""
+    import dataclasses
+    note(f"res_calc with repeated edges (cells of up to 8 increments), strategy={strategy}")
+    configure(strategy, 128)
+    base = M.o_mesh(48, 24, "random", seed=5)
+    mesh = dataclasses.replace(base, edge=np.ascontiguousarray(np.concatenate([base.edge, base.edge])),
+                               ecell=np.ascontiguousarray(np.concatenate([base.ecell, base.ecell])))
+    c = M.flow_constants().as_array()
+    rng = np.random.Generator(np.random.PCG64(8))
+    q0 = np.tile(c[6:10], (mesh.ncell, 1)) * (1 + 0.02 * rng.standard_normal((mesh.ncell, 4)))
+    b = AF.Airfoil(mesh, c, "double", q0=q0)
+    s = op2.plan_summary("res_calc", b.edges, b.res_calc_args(), strategy, 128)
+    assert s["verify_errors"] == 0 and s["max_incs"] == 8
+    oadt = np.zeros(mesh.ncell)
+    O.adt_calc("f64", mesh, mesh.x.copy(), q0.copy(), oadt, c)
+    ores = np.zeros((mesh.ncell, 4))
+    O.res_calc("f64", mesh, mesh.x.copy(), q0.copy(), oadt, ores, c)
+    b.adt_calc()
+    b.res_calc()
+    assert relmax(op2.op_fetch_data(b.p_res).reshape(-1, 4), ores) < 1e-13
+
+
+def test_online_autotuner_runs_real_iterations(runtime):
+    """Online tuning times each candidate inside real iterations, so the
+    whole run, tuning included, must match the oracle; graph capture is
+    refused until the tuning has finished."""
+    note("online autotune res_calc/bres_calc on 48x24 random, 30 iterations")
+    mesh = M.o_mesh(48, 24, "random", seed=6)
+    c = M.flow_constants().as_array()
+    a = AF.Airfoil(mesh, c, "double")
+    for n in ("res_calc", "bres_calc"):
+        op2.autotune_online(n, 2)
+    hist = [a.iterate()]
+    assert op2.autotune_online_result("res_calc") is None
+    op2.set_deferred_reductions(True)
+    with pytest.raises(op2.OP2Error, match="unfinished"):
+        op2.graph_begin()
+    op2.set_deferred_reductions(False)
+    hist += [a.iterate() for _ in range(29)]
+    r, b = op2.autotune_online_result("res_calc"), op2.autotune_online_result("bres_calc")
+    note(f"online choices: res_calc {r}, bres_calc {b}")
+    assert r["strategy"] in (1, 2, 4, 5) and b["strategy"] in (1, 2, 4)
+    _, oh = O.airfoil_seq(mesh, c, 30, "f64")
+    assert np.max(np.abs(np.array(hist) - oh) / oh) < TOL64
+
+
+def test_online_autotuner_jacobi_and_refusals(runtime, golden_dir):
+    note("online autotune of jac1 res; direct loop refused")
+    fx = np.load(os.path.join(golden_dir, "jacobi_64_random_s1_f64.npz"))
+    op2.autotune_online("res", 1)
+    prob = M.jacobi_grid(64, seed=1)
+    hs, _, u, _ = AF.run_jacobi(prob, int(fx["niter"]), 0.25, -0.25, "double")
+    assert op2.autotune_online_result("res") is not None
+    assert relmax(hs, fx["seq_u_sum"]) < TOL64
+    assert relmax(u, fx["seq_u"]) < TOL64
+    mesh = M.o_mesh(24, 12, "random", seed=1)
+    a = AF.Airfoil(mesh, M.flow_constants().as_array(), "double")
+    op2.autotune_online("save_soln", 1)
+    with pytest.raises(op2.OP2Error, match="cannot be autotuned"):
+        a.save_soln()

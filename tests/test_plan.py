@@ -33,6 +33,7 @@ def declare(mesh):
     pecell = op2.op_decl_map(edges, cells, 2, mesh.ecell, "pecell")
     pbedge = op2.op_decl_map(bedges, nodes, 2, mesh.bedge, "pbedge")
     pbecell = op2.op_decl_map(bedges, cells, 1, mesh.becell, "pbecell")
+    pcell = op2.op_decl_map(cells, nodes, 4, mesh.cell, "pcell")
     x = op2.op_decl_dat_host(nodes, 2, "double", mesh.x, "x")
     q = op2.op_decl_dat_host(cells, 4, "double", np.zeros(4 * mesh.ncell), "q")
     adt = op2.op_decl_dat_host(cells, 1, "double", np.zeros(mesh.ncell), "adt")
@@ -53,7 +54,9 @@ def declare(mesh):
                  op2.op_arg_dat(adt, 0, pbecell, 1, "double", R),
                  op2.op_arg_dat(res, 0, pbecell, 4, "double", I),
                  op2.op_arg_dat(bound, -1, None, 1, "int", R)]
-    return dict(edges=edges, bedges=bedges, res_args=res_args, bres_args=bres_args)
+    return dict(edges=edges, bedges=bedges, nodes=nodes, cells=cells, pedge=pedge, pecell=pecell,
+                pbedge=pbedge, pbecell=pbecell, pcell=pcell, res_args=res_args,
+                bres_args=bres_args)
 
 
 def check_plan(exp, targets, coloured):
@@ -147,6 +150,7 @@ def test_owner_plans(host_runtime, ordering, tile):
     d = declare(mesh)
     s = op2.plan_summary("res_calc", d["edges"], d["res_args"], op2.STRATEGY_OWNER, tile)
     assert s["verify_errors"] == 0
+    assert s["max_incs"] == 4  # quad cells: packed 4-entry records
     # every cell is owned once: about ncell / tile tiles, each within the limit
     assert s["nblocks"] >= mesh.ncell // tile
     assert s["max_block"] <= 3 * tile + 8
@@ -162,3 +166,63 @@ def test_owner_tiles_are_compact(host_runtime):
     s = op2.plan_summary("res_calc", d["edges"], d["res_args"], op2.STRATEGY_OWNER, 256)
     assert s["verify_errors"] == 0
     assert s["max_block"] < 2.5 * 256
+
+
+# ---------------------------------------------------------------------------
+# renumbering (host side: permutations, internal maps, compact owner tiles)
+
+@pytest.mark.parametrize("ordering", ["random", "cartesian"])
+def test_renumber_permutations_and_maps(host_runtime, ordering):
+    """Every set reached from the cells is permuted (a bijection), the
+    internal maps are the user's maps under the permutations, and owner
+    tiles of the renumbered cells are compact whatever the user ordering."""
+    mesh = M.o_mesh(64, 32, ordering, seed=3)
+    d = declare(mesh)
+    before = op2.plan_summary("res_calc", d["edges"], d["res_args"], op2.STRATEGY_OWNER, 128)
+    op2.renumber(d["pecell"])
+    perms = {}
+    for name in ("cells", "edges", "nodes", "bedges"):
+        p = op2.renumber_export(d[name])
+        assert p is not None and np.array_equal(np.sort(p), np.arange(d[name].size))
+        perms[name] = p
+    for mname, frm, to in (("pecell", "edges", "cells"), ("pedge", "edges", "nodes"),
+                           ("pcell", "cells", "nodes"), ("pbecell", "bedges", "cells")):
+        m = d[mname]
+        core = C.cast(m._p, C.POINTER(op2._MapCore)).contents
+        n = d[frm].size
+        internal = np.ctypeslib.as_array(C.cast(core.map_i, C.POINTER(C.c_int)), (n * m.dim,))
+        user = getattr(mesh, {"pecell": "ecell", "pedge": "edge", "pcell": "cell",
+                              "pbecell": "becell"}[mname]).reshape(n, -1)
+        want = np.empty_like(user)
+        want[perms[frm]] = perms[to][user]
+        assert np.array_equal(internal.reshape(n, -1), want), mname
+    s = op2.plan_summary("res_calc", d["edges"], d["res_args"], op2.STRATEGY_OWNER, 128)
+    assert s["verify_errors"] == 0 and s["max_incs"] == 4
+    # aligned runs of 128 renumbered cells: full tiles with few cut edges
+    # (an 11 x 11 square of cells touches 2.2 x 128 edges)
+    assert s["nblocks"] == -(-mesh.ncell // 128) <= before["nblocks"]
+    assert s["max_block"] < 2.6 * 128
+    g = op2.plan_summary("res_calc", d["edges"], d["res_args"], op2.STRATEGY_OWNER_GATHER, 256)
+    assert g["verify_errors"] == 0
+    c = op2.plan_summary("res_calc", d["edges"], d["res_args"], op2.STRATEGY_COLOURED, 256)
+    assert c["verify_errors"] == 0
+
+
+def test_renumber_refusals(host_runtime):
+    mesh = M.o_mesh(16, 8, "random", seed=1)
+    d = declare(mesh)
+    op2.renumber(d["pecell"])
+    with pytest.raises(op2.OP2Error, match="already renumbered"):
+        op2.renumber(d["pecell"])
+
+
+def test_renumber_keeps_components_in_order(host_runtime):
+    """Disconnected meshes (a batch) stay contiguous and in their order, so
+    a cell -> mesh map stays non-decreasing."""
+    b = M.mesh_batch(nmesh=4, seed=2, cells_lo=200, cells_hi=900, ordering="random")
+    d = declare(b.union)
+    op2.renumber(d["pecell"])
+    p = op2.renumber_export(d["cells"])
+    new_mesh = np.empty(b.union.ncell, np.int64)
+    new_mesh[p] = b.cmesh
+    assert np.all(np.diff(new_mesh) >= 0)
