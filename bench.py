@@ -12,6 +12,8 @@ roofline: res_calc, the hot loop.  achieved = algorithmic bytes of one call
 (each touched dat once per direction plus each map once: 76 B/edge FP64)
 divided by the mean device time of one call, from HIP events the runtime
 records on its stream around every loop call of an eager timed region.
+traffic = measured HBM bytes per call from the committed rocprofv3 counter
+summary, when this run's configuration is the profiled one.
 
 fp32_form: the same mesh with float state and flux (update and rms in
 double): its throughput, speed-up over FP64, and its error against the FP64
@@ -66,6 +68,25 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="target CPU time of the cpu_baseline sample (0 = skip)")
     return ap.parse_args()
+
+
+TRAFFIC_PROFILE = os.path.join("profiles", "airfoil_2400x1200_fp64_traffic.md")
+
+
+def measured_traffic(ni, nj, res_cfg):
+    """HBM bytes per res_calc launch from the committed rocprofv3 counter
+    summary (FETCH_SIZE x copy-kernel factor + WRITE_SIZE), when this run's
+    configuration is the one profiled (2400x1200 FP64, renumbered,
+    owner-gather with 256-cell tiles); else None."""
+    path = os.path.join(ROOT, TRAFFIC_PROFILE)
+    if (ni, nj) != (2400, 1200) or not res_cfg or not os.path.exists(path):
+        return None
+    if (res_cfg.get("strategy"), res_cfg.get("part_size")) != (5, 256):
+        return None
+    for line in open(path):
+        if line.startswith("| `op2::k_res_calc_owner_direct<double"):
+            return float(line.split("|")[7]) * 1e6
+    return None
 
 
 def measure(mesh, consts, dtype, a):
@@ -210,6 +231,9 @@ def main():
     r64 = measure(mesh, consts, "double", a)
     rc = r64["loops"]["res_calc"]
     achieved = rc["GBs"]
+    res_cfg = (r64["tuned"] or {}).get("res_calc") if a.strategy.startswith("tune") else \
+        {"strategy": STRATEGIES[a.strategy], "part_size": a.part}
+    traffic = measured_traffic(ni, nj, res_cfg) if a.renumber == "on" else None
     result = {
         "metric": METRIC,
         "value": r64["value"],
@@ -231,7 +255,8 @@ def main():
                    "renumbered": a.renumber == "on",
                    "tuned": r64["tuned"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": TRAFFIC_PROFILE if traffic else None,
                      "kernel": "res_calc", "bytes_per_launch": rc["bytes_per_call"],
                      "bytes_per_edge": rc["bytes_per_call"] / mesh.nedge,
                      "us_per_launch": rc["us_per_call"]},
