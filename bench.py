@@ -152,6 +152,15 @@ def measure_batch(a):
                  af.autotune() if a.strategy == "tune-offline" else None)
         for _ in range(a.warmup):
             af.iterate()
+        op2.sync()
+        op2.timing_reset()
+        for _ in range(10):
+            af.iterate()
+        loops = {}
+        for name in ("save_soln", "adt_calc", "res_calc", "bres_calc", "update"):
+            st = op2.loop_stats(name)
+            if st:
+                loops[name] = {"calls": st[0], "us_per_call": 1e6 * st[1] / st[0]}
         gid = af.capture_iteration()
         af.run_graph(gid, a.warmup)
         op2.sync()
@@ -169,7 +178,7 @@ def measure_batch(a):
             "cells_min": int(min(m.ncell for m in batch.meshes)),
             "cells_max": int(max(m.ncell for m in batch.meshes)),
             "rms_last_min": float(hist[-1].min()), "rms_last_max": float(hist[-1].max()),
-            "tuned": tuned}
+            "tuned": tuned, "loops": loops}
 
 
 def accuracy(mesh, consts, iters):

@@ -566,8 +566,12 @@ __global__ __launch_bounds__(NT) void k_update_seg(const T *__restrict__ qold, T
                                                    int ncell, const int *__restrict__ cmap,
                                                    const int *__restrict__ seg_off,
                                                    const int *__restrict__ pslot,
-                                                   double *__restrict__ partials) {
+                                                   double *__restrict__ partials,
+                                                   unsigned *__restrict__ log_count, int nseg,
+                                                   unsigned *__restrict__ log_base) {
   __shared__ double lds[NT / kWave];
+  // reserve the log slots of this call's nseg results for k_fold_seg
+  if (log_count && blockIdx.x == 0 && threadIdx.x == 0) *log_base = atomicAdd(log_count, (unsigned)nseg);
   constexpr int CH = NT * UPC;
   const int c_lo = blockIdx.x * CH;
   const int c_hi = min(c_lo + CH, ncell);
@@ -607,22 +611,23 @@ __global__ __launch_bounds__(NT) void k_update_seg(const T *__restrict__ qold, T
   }
 }
 
-// Per mesh m: out = sum of its partials in workgroup order; the dat gets
-// dat[m] += out and, with deferred reductions, the log gets out for every
-// mesh in mesh order.
+// One workgroup per mesh m: out = sum of its partials (thread-strided, then
+// the block tree: a fixed order); the dat gets dat[m] += out and, with
+// deferred reductions, the log slot reserved by k_update_seg gets out (the
+// results of one call in mesh order).
 __global__ __launch_bounds__(NT) void k_fold_seg(const double *__restrict__ partials,
-                                                 const int *__restrict__ pslot, int nseg,
+                                                 const int *__restrict__ pslot,
                                                  double *__restrict__ dat,
                                                  double *__restrict__ log,
-                                                 unsigned *__restrict__ log_count) {
-  __shared__ unsigned base;
-  if (log && threadIdx.x == 0) base = atomicAdd(log_count, (unsigned)nseg);
-  __syncthreads();
-  for (int m = threadIdx.x; m < nseg; m += NT) {
-    double acc = 0.0;
-    for (int i = pslot[m]; i < pslot[m + 1]; i++) acc += partials[i];
+                                                 const unsigned *__restrict__ log_base) {
+  __shared__ double lds[NT / kWave];
+  const int m = blockIdx.x;
+  double acc = 0.0;
+  for (int i = pslot[m] + threadIdx.x; i < pslot[m + 1]; i += NT) acc += partials[i];
+  acc = block_reduce<kRedSum, NT>(acc, lds);
+  if (threadIdx.x == 0) {
     dat[m] += acc;
-    if (log) log[(base + m) & (kLogCap - 1)] = acc;
+    if (log) log[(*log_base + m) & (kLogCap - 1)] = acc;
   }
 }
 
@@ -1134,6 +1139,7 @@ struct SegPlan {
   int nseg = 0, nslots = 0;
   int *d_seg_off = nullptr, *d_pslot = nullptr;
   double *d_partials = nullptr;
+  unsigned *d_log_base = nullptr;  // log slot of this map's results in the current call
 };
 
 constexpr int kSegUpc = 4;  // cells per thread in k_update_seg
@@ -1169,6 +1175,7 @@ SegPlan &seg_plan(op_map m) {
   sp.d_pslot = (int *)dev_alloc(sizeof(int) * pslot.size());
   h2d(sp.d_pslot, pslot.data(), sizeof(int) * pslot.size());
   sp.d_partials = (double *)dev_alloc(sizeof(double) * std::max(sp.nslots, 1));
+  sp.d_log_base = (unsigned *)dev_alloc(256);
   cache.push_back(sp);
   return cache.back();
 }
@@ -1181,6 +1188,7 @@ void release_exec_caches() {
     dev_free(sp.d_seg_off);
     dev_free(sp.d_pslot);
     dev_free(sp.d_partials);
+    dev_free(sp.d_log_base);
   }
   seg_cache().clear();
 }
@@ -1197,10 +1205,13 @@ void run_update_batch(op_set set, op_arg *a) {
   hipLaunchKernelGGL(kern, dim3(grid), dim3(NT), 0, r.stream,
                      (const T *)a[0].dat->data_d, (T *)a[1].dat->data_d, (T *)a[2].dat->data_d,
                      (const T *)a[3].dat->data_d, set->size, (const int *)a[4].map->map_d,
-                     (const int *)sp.d_seg_off, (const int *)sp.d_pslot, sp.d_partials);
-  hipLaunchKernelGGL(k_fold_seg, dim3(1), dim3(NT), 0, r.stream, (const double *)sp.d_partials,
-                     (const int *)sp.d_pslot, sp.nseg, (double *)a[4].dat->data_d,
-                     r.deferred ? r.d_log : nullptr, r.d_log_count);
+                     (const int *)sp.d_seg_off, (const int *)sp.d_pslot, sp.d_partials,
+                     r.deferred ? r.d_log_count : nullptr, sp.nseg, sp.d_log_base);
+  if (sp.nseg > 0)
+    hipLaunchKernelGGL(k_fold_seg, dim3(sp.nseg), dim3(NT), 0, r.stream,
+                       (const double *)sp.d_partials, (const int *)sp.d_pslot,
+                       (double *)a[4].dat->data_d, r.deferred ? r.d_log : nullptr,
+                       (const unsigned *)sp.d_log_base);
 }
 
 template <typename T>
