@@ -17,6 +17,8 @@
 
 #include "op_lib_core.h"
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -195,6 +197,48 @@ int op2_renumber(op_map base);
  * be NULL): returns 1 if the set was renumbered, 0 if its numbering is the
  * user's. */
 int op2_renumber_export(op_set set, int *new_of_old);
+
+/* ---- generic loop path (no counterpart function in the reference; it
+ * takes the place of the code common/op2.m:142-1141 generates per loop) ----
+ * A translation unit compiled with hipcc instantiates the header-only
+ * executor of include/op_hip_loop.h for the user's kernels and registers
+ * each loop by name; op_par_loop / op2_par_loop then run a registered loop
+ * whenever no built-in kernel matches the loop's name and arguments. */
+typedef int (*op2_loop_fn)(char const *name, op_set set, int nargs, op_arg *args);
+int op2_register_loop(char const *name, op2_loop_fn fn);
+
+/* Device copy of a constant: called with the constant's bytes and the
+ * runtime stream whenever op_decl_const / op_decl_const2 declares `name`
+ * (and at registration if it is already declared). */
+typedef void (*op2_const_copy_fn)(void const *data, size_t bytes, void *stream);
+int op2_register_const(char const *name, op2_const_copy_fn fn);
+
+/* Execution arrays of the loop's generic plan (built and verified on first
+ * use): contiguous blocks of `part_size` elements (0 = 256) whose elements
+ * are sorted by thread colour, thread colours and block colours such that
+ * no two elements running together write the same indirect target (every
+ * access other than OP_READ counts).  Pointers are device arrays except
+ * colblk_off ([ncolors+1] on the host). */
+typedef struct {
+  int nblocks, ncolors;
+  int const *blk_off, *perm, *col_off, *col_base, *blk_ncol, *blkmap;
+  int const *colblk_off;
+} op2_generic_plan;
+int op2_generic_plan_get(char const *name, op_set set, int nargs, op_arg *args, int part_size,
+                         op2_generic_plan *out);
+/* strategy value under which op2_plan_summary_get reports the generic plan */
+#define OP2_PLAN_GENERIC 16
+
+/* Reports an error of a generic loop through the runtime's error mode
+ * (prints and exits, or records it for op2_last_error); returns -1. */
+int op2_report_error(char const *msg);
+
+/* Device copy of a map in the runtime's numbering (uploaded on first use). */
+int const *op2_map_device_ptr(op_map map);
+
+/* Runtime-owned device scratch of at least `bytes` (valid until the next
+ * call that asks for more). */
+void *op2_device_scratch(size_t bytes);
 
 /* Device seconds, call count and algorithmic bytes per call of a loop
  * (device time from HIP events on the runtime stream).  Returns 0 if the

@@ -413,6 +413,11 @@ void op2_decl_const_char(int dim, char const *type, int size, char *data, char c
   c.size = size;
   c.bytes.assign(data, data + (size_t)dim * size);
   rt().consts[name] = c;
+  auto it = rt().const_targets.find(name);
+  if (it != rt().const_targets.end()) {
+    ensure_device();
+    it->second(c.bytes.data(), c.bytes.size(), (void *)rt().stream);
+  }
   API_END()
 }
 
@@ -757,6 +762,68 @@ int op2_autotune_online_result(char const *name, int *best_strategy, int *best_p
   if (best_us) *best_us = it->second.best_us;
   return 1;
   API_END(-1)
+}
+
+int op2_register_loop(char const *name, op2_loop_fn fn) {
+  API_BEGIN
+  if (!name || !fn) fail("op2_register_loop: invalid arguments");
+  rt().loops[name] = fn;
+  return 0;
+  API_END(-1)
+}
+
+int op2_register_const(char const *name, op2_const_copy_fn fn) {
+  API_BEGIN
+  if (!name || !fn) fail("op2_register_const: invalid arguments");
+  Runtime &r = rt();
+  r.const_targets[name] = fn;
+  auto it = r.consts.find(name);
+  if (it != r.consts.end()) {
+    ensure_device();
+    fn(it->second.bytes.data(), it->second.bytes.size(), (void *)r.stream);
+  }
+  return 0;
+  API_END(-1)
+}
+
+int op2_generic_plan_get(char const *name, op_set set, int nargs, op_arg *args, int part_size,
+                         op2_generic_plan *out) {
+  API_BEGIN
+  if (!name || !set || nargs <= 0 || !args || !out) fail("op2_generic_plan_get: invalid arguments");
+  PlanRequest req{name, set, nargs, args, kStrategyGeneric, part_size > 0 ? part_size : 256};
+  Plan *p = get_plan(req);
+  p->upload();
+  out->nblocks = p->nblocks;
+  out->ncolors = p->ncolors;
+  out->blk_off = p->d_blk_off;
+  out->perm = p->d_perm;
+  out->col_off = p->d_col_off;
+  out->col_base = p->d_col_base;
+  out->blk_ncol = p->d_blk_ncol;
+  out->blkmap = p->d_blkmap;
+  out->colblk_off = p->colblk_off.data();
+  return 0;
+  API_END(-1)
+}
+
+int op2_report_error(char const *msg) {
+  API_BEGIN
+  fail("%s", msg ? msg : "op2: error");
+  API_END(-1)
+}
+
+int const *op2_map_device_ptr(op_map map) {
+  API_BEGIN
+  if (!map) fail("op2_map_device_ptr: invalid map");
+  ensure_map_device(map);
+  return map->map_d;
+  API_END(nullptr)
+}
+
+void *op2_device_scratch(size_t bytes) {
+  API_BEGIN
+  return reduct_scratch(bytes);
+  API_END(nullptr)
 }
 
 int op2_renumber(op_map base) {

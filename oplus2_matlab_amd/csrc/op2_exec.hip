@@ -1480,9 +1480,21 @@ int par_loop(const char *name, op_set set, int nargs, op_arg *args) {
     if (args[m].map) ensure_map_device(args[m].map);
   }
   const Loop l = identify(name, set, nargs, args);
-  if (l == Loop::kUnknown)
-    fail("op_par_loop error -- no device kernel for loop \"%s\" with this argument signature",
-         name);
+  if (l == Loop::kUnknown) {
+    // a loop registered by a translation unit of the generic path
+    auto it = rt().loops.find(name);
+    if (it == rt().loops.end())
+      fail("op_par_loop error -- no device kernel for loop \"%s\" with this argument "
+           "signature, and no registered generic loop of that name", name);
+    LoopTiming &t = timing_for(name, loop_bytes(set, nargs, args));
+    hipEvent_t start;
+    timing_begin(t, &start);
+    const int rc = it->second(name, set, nargs, args);
+    if (rc != 0) fail("op_par_loop error -- generic loop \"%s\": %s", name, rt().last_error.c_str());
+    check_launch(name);
+    timing_end(t, start);
+    return 0;
+  }
   // the batched update's per-mesh rms is double whatever the state type
   const int rb = real_bytes(l == Loop::kUpdateBatch ? 4 : nargs, args);
   if (rb != 4 && rb != 8)

@@ -183,6 +183,10 @@ struct Runtime {
   std::map<op_set, std::vector<int>> set_perm;
   std::map<op_set, std::vector<int>> set_segments;
   std::map<op_map, std::vector<int>> map_internal;
+  // generic loop path: loops and device constants registered by the
+  // translation units that compile the user kernels
+  std::map<std::string, op2_loop_fn> loops;
+  std::map<std::string, op2_const_copy_fn> const_targets;
 };
 
 constexpr unsigned kLogCap = 1u << 16;  // power of two
@@ -206,6 +210,10 @@ void permute_to_user(op_dat d, const char *internal, char *user);
 
 const Const &get_const(const char *name);
 LoopConfig config_for(const std::string &loop);
+
+// plan of the generic loop path (header op_hip_loop.h): coloured on every
+// indirect write, no staging lists; not selectable by op2_set_loop_config
+constexpr int kStrategyGeneric = OP2_PLAN_GENERIC;
 
 // plan construction (op2_plan.cpp)
 struct PlanRequest {

@@ -43,7 +43,7 @@ void Plan::upload() {
   uploaded = true;
   auto up = [](const void *src, size_t bytes) -> void * {
     void *d = dev_alloc(bytes);
-    h2d(d, src, bytes);
+    if (src) h2d(d, src, bytes);  // (an empty vector has no data: nothing to copy)
     return d;
   };
   d_blk_off = (int *)up(blk_off.data(), sizeof(int) * blk_off.size());
@@ -300,6 +300,7 @@ static Plan *build(const PlanRequest &req) {
   }
 
   // groups (distinct maps) and slots (distinct (map, idx))
+  const bool generic = req.strategy == kStrategyGeneric;
   std::vector<int> inc_slots;
   P.arg_slot.assign(req.nargs, -1);
   bool has_direct_dat = false;
@@ -326,13 +327,17 @@ static Plan *build(const PlanRequest &req) {
       s = (int)P.slots.size() - 1;
     }
     P.arg_slot[a] = s;
-    if (arg.acc == OP_INC && std::find(inc_slots.begin(), inc_slots.end(), s) == inc_slots.end())
+    // the generic path colours on every indirect write (its kernels write
+    // to global memory directly), the device kernels on increments
+    const bool writes = arg.acc == OP_INC || (generic && arg.acc != OP_READ);
+    if (writes && std::find(inc_slots.begin(), inc_slots.end(), s) == inc_slots.end())
       inc_slots.push_back(s);
   }
   const int nslots = (int)P.slots.size();
   if (nslots == 0) fail("op2: plan requested for loop %s without indirect arguments", req.loop.c_str());
-  if (nslots > 4) fail("op2: loop %s has %d (map, index) pairs; at most 4 are supported", req.loop.c_str(), nslots);
-  P.slot_stride = nslots == 1 ? 1 : nslots == 2 ? 2 : 4;
+  if (nslots > 4 && !generic)
+    fail("op2: loop %s has %d (map, index) pairs; at most 4 are supported", req.loop.c_str(), nslots);
+  P.slot_stride = generic ? 0 : nslots == 1 ? 1 : nslots == 2 ? 2 : 4;
 
   const int bsize = req.part_size > 0 ? req.part_size : 512;
   const bool owner = req.strategy == OP2_STRATEGY_OWNER || req.strategy == OP2_STRATEGY_OWNER_GATHER;
@@ -356,7 +361,8 @@ static Plan *build(const PlanRequest &req) {
   }
 
   // thread colours inside each block, then positions sorted by colour
-  const bool coloured = req.strategy == OP2_STRATEGY_COLOURED && !inc_slots.empty();
+  const bool coloured =
+      (req.strategy == OP2_STRATEGY_COLOURED || generic) && !inc_slots.empty();
   if (coloured) {
     P.perm.resize(n);
     P.thrcol.resize(n);
@@ -399,7 +405,7 @@ static Plan *build(const PlanRequest &req) {
   // per block and group: distinct targets (ascending) and local indices
   P.loc.assign(npos * P.slot_stride, 0);
   std::vector<int> buf;
-  for (int b = 0; b < P.nblocks; b++) {
+  for (int b = 0; b < P.nblocks && !generic; b++) {
     const int lo = P.blk_off[b], hi = P.blk_off[b + 1];
     for (int g = 0; g < ng; g++) {
       PlanGroup &G = P.groups[g];
@@ -456,8 +462,9 @@ static Plan *build(const PlanRequest &req) {
         P.ids[pos * P.slot_stride + s] =
             target(P.groups[P.slots[s].first].map, P.slots[s].second, elem((int)pos));
   }
-  // kernels read perm only for direct dats, and only if elements moved
-  P.need_perm = has_direct_dat && !P.perm.empty();
+  // kernels read perm only for direct dats, and only if elements moved (the
+  // generic kernels take every element through it)
+  P.need_perm = (has_direct_dat || generic) && !P.perm.empty();
   if (owner) P.mean_thread_colours = 0;
 
   double tv0, tv1, cpu;
@@ -600,7 +607,7 @@ long long verify_plan(const Plan &P) {
         continue;
       }
       seen[e] = 1;
-      for (int s = 0; s < nslots; s++) {
+      for (int s = 0; s < nslots && P.strategy != kStrategyGeneric; s++) {
         const PlanGroup &G = P.groups[P.slots[s].first];
         int l = P.loc[(size_t)pos * P.slot_stride + s];
         int cnt = G.off[b + 1] - G.off[b];
@@ -615,12 +622,13 @@ long long verify_plan(const Plan &P) {
     for (int b = 0; b < P.nblocks; b++)
       for (int k = G.off[b] + 1; k < G.off[b + 1]; k++) err += G.list[k] <= G.list[k - 1];
 
+  const bool generic = P.strategy == kStrategyGeneric;
   std::vector<int> inc_slots;
   for (size_t a = 0; a < P.maps.size(); a++)
-    if (P.accs[a] == OP_INC && P.arg_slot[a] >= 0 &&
+    if ((P.accs[a] == OP_INC || (generic && P.accs[a] != OP_READ)) && P.arg_slot[a] >= 0 &&
         std::find(inc_slots.begin(), inc_slots.end(), P.arg_slot[a]) == inc_slots.end())
       inc_slots.push_back(P.arg_slot[a]);
-  if (P.strategy != OP2_STRATEGY_COLOURED || inc_slots.empty()) return err;
+  if ((P.strategy != OP2_STRATEGY_COLOURED && !generic) || inc_slots.empty()) return err;
 
   // (group, target, colour, owner) clash search
   struct Q {
@@ -736,7 +744,8 @@ int op2_plan_summary_get(char const *name, op_set set, int nargs, op_arg *args, 
                          int part_size, op2_plan_summary *out) {
   try {
     if (strategy != OP2_STRATEGY_COLOURED && strategy != OP2_STRATEGY_ATOMIC &&
-        strategy != OP2_STRATEGY_OWNER && strategy != OP2_STRATEGY_OWNER_GATHER)
+        strategy != OP2_STRATEGY_OWNER && strategy != OP2_STRATEGY_OWNER_GATHER &&
+        strategy != OP2_PLAN_GENERIC)
       fail("op2: plan summary needs an explicit strategy");
     PlanRequest req{name, set, nargs, args, strategy, part_size};
     Plan *p = get_plan(req);

@@ -26,6 +26,7 @@ OP_ARG_GBL, OP_ARG_DAT = 0, 1
 OP_ID = None
 OP_GBL = None
 
+PLAN_GENERIC = 16  # plan of the generic loop path (op2_plan_summary_get only)
 STRATEGY_AUTO, STRATEGY_COLOURED, STRATEGY_ATOMIC, STRATEGY_DIRECT, STRATEGY_OWNER, STRATEGY_OWNER_GATHER = 0, 1, 2, 3, 4, 5
 ERRORS_EXIT, ERRORS_RETURN = 0, 1
 
@@ -98,6 +99,12 @@ SIGNATURES = {
     "op2_last_error": (C.c_char_p, []),
     "op2_clear_error": (None, []),
     "op2_set_loop_config": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int]),
+    "op2_register_loop": (C.c_int, [C.c_char_p, _P]),
+    "op2_register_const": (C.c_int, [C.c_char_p, _P]),
+    "op2_generic_plan_get": (C.c_int, [C.c_char_p, _P, C.c_int, C.POINTER(OpArg), C.c_int, _P]),
+    "op2_map_device_ptr": (_P, [_P]),
+    "op2_device_scratch": (_P, [C.c_size_t]),
+    "op2_report_error": (C.c_int, [C.c_char_p]),
     "op2_renumber": (C.c_int, [_P]),
     "op2_renumber_export": (C.c_int, [_P, _P]),
     "op2_autotune_online": (C.c_int, [C.c_char_p, C.c_int]),

@@ -181,6 +181,10 @@ def jacobi(tmp):
     out = run([os.path.join(REFBIN, "jac_ref")])
     with open(os.path.join(GOLD, "jac1_ref_nn6.txt"), "w") as f:
         f.write(out)
+    # jac2: the same problem with multi-component, mixed-type dats
+    out = run([os.path.join(REFBIN, "jac2_ref")])
+    with open(os.path.join(GOLD, "jac2_ref_nn6.txt"), "w") as f:
+        f.write(out)
 
 
 def plans(tmp):

@@ -68,3 +68,18 @@ def test_unchanged_jac1_program(tmp_path, golden_dir, prog):
     table = ref[ref.index("Results after"):].splitlines()[2:7]
     for ln in table:
         assert ln in out
+
+
+@pytest.mark.parametrize("prog", ["jac2", "jac2_op"])
+def test_unchanged_jac2_program_on_the_generic_path(tmp_path, golden_dir, prog):
+    """jac2 (multi-component, mixed float/double dats) has no built-in
+    kernel: its loops run on the generic path, compiled from the unchanged
+    jac2/res.h and jac2/update.h (csrc/dropin/jac2_kernels.hip)."""
+    exe = need(prog)
+    out = run(exe, tmp_path)
+    ref = open(os.path.join(golden_dir, "jac2_ref_nn6.txt")).read()
+    want = [ln for ln in ref.splitlines() if "u max/rms" in ln]
+    assert len(want) == 2 and all(ln in out for ln in want)
+    table = ref[ref.index("Results after"):].splitlines()[2:7]
+    for ln in table:
+        assert ln in out

@@ -1,0 +1,184 @@
+"""GPU tests of the generic loop path (include/op_hip_loop.h): loops the
+runtime has no built-in kernel for, compiled from device user functions in
+tests/generic_loops.hip and registered by name, checked against the
+sequential semantics of op_par_loop (common/op_seq.h) restated in numpy.
+Exact where the result does not depend on the order of the elements
+(writes, RW maxima, MIN/MAX reductions), within float rounding for sums."""
+
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+from oplus2_matlab_amd import mesh as M
+from oplus2_matlab_amd import op2
+from oplus2_matlab_amd.op2 import (OP_ID, OP_INC, OP_MAX, OP_MIN, OP_READ, OP_RW, OP_WRITE,
+                                   op_arg_dat, op_arg_gbl)
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.abspath(os.path.join(os.path.dirname(__file__), ".."))
+LOOPS = os.path.join(ROOT, "oplus2_matlab_amd", "lib", "libop2_testloops.so")
+
+
+def note(msg):
+    print(f"[gpu] {msg}", flush=True)
+
+
+@pytest.fixture(scope="module")
+def loops():
+    if not os.path.exists(LOOPS):
+        pytest.fail(f"{LOOPS} not built (make -C oplus2_matlab_amd/csrc testloops)")
+    op2.lib()
+    return C.CDLL(LOOPS, mode=C.RTLD_GLOBAL)  # registers gen_direct, gen_edge, gen_cell
+
+
+@pytest.fixture
+def runtime(loops):
+    op2.op_init([], 0)
+    yield
+    op2.op_exit()
+
+
+def test_direct_loop_every_global_access(runtime):
+    note("generic direct loop: mixed types, dims 3/2, OP_GBL READ/INC/MAX/MIN")
+    n = 100_003
+    rng = np.random.Generator(np.random.PCG64(4))
+    a = np.ascontiguousarray(rng.random((n, 3)))
+    b = np.zeros((n, 2), np.float32)
+    cells = op2.op_decl_set(n, "cells")
+    pa = op2.op_decl_dat(cells, 3, "double", a, "pa")
+    pb = op2.op_decl_dat(cells, 2, "float", b, "pb")
+    op2.op_decl_const2("gen_scale", 1, "double", np.array([2.0]))
+    s = np.array([1.5, -0.5])
+    tot = np.array([0.25, 1.0], np.float32)
+    imax = np.array([-5], np.int32)
+    dmin = np.array([10.0])
+    op2.op_par_loop(None, "gen_direct", cells,
+                    op_arg_dat(pa, -1, OP_ID, 3, "double", OP_READ),
+                    op_arg_dat(pb, -1, OP_ID, 2, "float", OP_WRITE),
+                    op_arg_gbl(s, 2, "double", OP_READ),
+                    op_arg_gbl(tot, 2, "float", OP_INC),
+                    op_arg_gbl(imax, 1, "int", OP_MAX),
+                    op_arg_gbl(dmin, 1, "double", OP_MIN))
+    b0 = (s[0] * a[:, 0] + a[:, 1]).astype(np.float32)
+    b1 = (s[1] * a[:, 2] * 2.0).astype(np.float32)
+    got = op2.op_fetch_data(pb).reshape(n, 2)
+    assert np.array_equal(got[:, 0], b0) and np.array_equal(got[:, 1], b1)
+    want = np.array([0.25 + b0.astype(np.float64).sum(),
+                     1.0 + (b1.astype(np.float64) ** 2).sum()])
+    np.testing.assert_allclose(tot, want, rtol=2e-6)
+    assert imax[0] == max(-5, int((a[:, 0] * 100.0).astype(np.int32).max()))
+    assert dmin[0] == min(10.0, float((a[:, 1] - a[:, 2]).min()))
+
+
+@pytest.mark.parametrize("renumber", [False, True])
+def test_indirect_increments(runtime, renumber):
+    note(f"generic edge loop: float dim-3 increments through both map indices, renumber={renumber}")
+    mesh = M.o_mesh(120, 60, "random", seed=3)
+    rng = np.random.Generator(np.random.PCG64(5))
+    nodes = op2.op_decl_set(mesh.nnode, "nodes")
+    edges = op2.op_decl_set(mesh.nedge, "edges")
+    pedge = op2.op_decl_map(edges, nodes, 2, mesh.edge, "pedge")
+    x = np.ascontiguousarray(mesh.x)
+    nd = np.ascontiguousarray(rng.random((mesh.nnode, 3)).astype(np.float32))
+    nd[:, 2] = 0.0  # counts the edges at each node: small integers, exact
+    nd0 = nd.copy()
+    w = np.ascontiguousarray(rng.random(mesh.nedge).astype(np.float32))
+    ln = np.zeros(mesh.nedge)
+    px = op2.op_decl_dat(nodes, 2, "double", x, "px")
+    pn = op2.op_decl_dat(nodes, 3, "float", nd, "pn")
+    pw = op2.op_decl_dat(edges, 1, "float", w, "pw")
+    pl = op2.op_decl_dat(edges, 1, "double", ln, "pl")
+    if renumber:
+        op2.renumber(pedge)
+    args = (op_arg_dat(px, 0, pedge, 2, "double", OP_READ),
+            op_arg_dat(px, 1, pedge, 2, "double", OP_READ),
+            op_arg_dat(pn, 0, pedge, 3, "float", OP_INC),
+            op_arg_dat(pn, 1, pedge, 3, "float", OP_INC),
+            op_arg_dat(pw, -1, OP_ID, 1, "float", OP_READ),
+            op_arg_dat(pl, -1, OP_ID, 1, "double", OP_WRITE))
+    for _ in range(3):
+        op2.op_par_loop(None, "gen_edge", edges, *args)
+    e1, e2 = mesh.edge[:, 0], mesh.edge[:, 1]
+    dx = x[e2, 0] - x[e1, 0]
+    dy = x[e2, 1] - x[e1, 1]
+    want = nd0.astype(np.float64)
+    for _ in range(3):
+        np.add.at(want[:, 0], e1, w.astype(np.float64))
+        np.add.at(want[:, 1], e1, dx.astype(np.float32).astype(np.float64))
+        np.add.at(want[:, 2], e1, 1.0)
+        np.add.at(want[:, 0], e2, -w.astype(np.float64))
+        np.add.at(want[:, 1], e2, -dy.astype(np.float32).astype(np.float64))
+        np.add.at(want[:, 2], e2, 1.0)
+    got = op2.op_fetch_data(pn).reshape(-1, 3)
+    np.testing.assert_allclose(got, want, rtol=0, atol=2e-5)
+    assert np.array_equal(got[:, 2], want[:, 2].astype(np.float32))
+    np.testing.assert_allclose(op2.op_fetch_data(pl), np.sqrt(dx * dx + dy * dy), rtol=1e-15)
+
+
+@pytest.mark.parametrize("renumber", [False, True])
+def test_indirect_rw_min_max_six_map_slots(runtime, renumber):
+    note(f"generic cell loop: OP_RW x4, OP_MAX/OP_MIN through a second map, renumber={renumber}")
+    mesh = M.o_mesh(96, 48, "random", seed=8)
+    rng = np.random.Generator(np.random.PCG64(6))
+    ng = 101
+    nodes = op2.op_decl_set(mesh.nnode, "nodes")
+    cells = op2.op_decl_set(mesh.ncell, "cells")
+    groups = op2.op_decl_set(ng, "groups")
+    pcell = op2.op_decl_map(cells, nodes, 4, mesh.cell, "pcell")
+    cg = np.ascontiguousarray(rng.integers(0, ng, (mesh.ncell, 2)).astype(np.int32))
+    pcg = op2.op_decl_map(cells, groups, 2, cg, "pcg")
+    c = np.ascontiguousarray((rng.random(mesh.ncell) * 50).astype(np.float32))
+    v = np.ascontiguousarray(rng.random(mesh.nnode) * 40)
+    v0 = v.copy()
+    lo_init = np.full(ng, 300, np.int32)
+    hi = np.zeros(ng, np.int32)
+    lo = lo_init.copy()
+    wc = np.zeros(mesh.ncell, np.float32)
+    pc = op2.op_decl_dat(cells, 1, "float", c, "pc")
+    pv = op2.op_decl_dat(nodes, 1, "double", v, "pv")
+    phi = op2.op_decl_dat(groups, 1, "int", hi, "phi")
+    plo = op2.op_decl_dat(groups, 1, "int", lo, "plo")
+    pwc = op2.op_decl_dat(cells, 1, "float", wc, "pwc")
+    if renumber:
+        op2.renumber(pcell)
+    op2.op_par_loop(None, "gen_cell", cells,
+                    op_arg_dat(pc, -1, OP_ID, 1, "float", OP_READ),
+                    *[op_arg_dat(pv, k, pcell, 1, "double", OP_RW) for k in range(4)],
+                    op_arg_dat(phi, 0, pcg, 1, "int", OP_MAX),
+                    op_arg_dat(plo, 1, pcg, 1, "int", OP_MIN),
+                    op_arg_dat(pwc, -1, OP_ID, 1, "float", OP_WRITE))
+    t = c.astype(np.float64)
+    want_v = v0.copy()
+    for k in range(4):
+        np.maximum.at(want_v, mesh.cell[:, k], t + k)
+    want_hi = np.zeros(ng, np.int32)
+    want_lo = lo_init.copy()
+    np.maximum.at(want_hi, cg[:, 0], (t * 10.0).astype(np.int32))
+    np.minimum.at(want_lo, cg[:, 1], (t * 10.0).astype(np.int32))
+    assert np.array_equal(op2.op_fetch_data(pv), want_v)
+    assert np.array_equal(op2.op_fetch_data(phi), want_hi)
+    assert np.array_equal(op2.op_fetch_data(plo), want_lo)
+    assert np.array_equal(op2.op_fetch_data(pwc), c)
+
+
+def test_generic_errors(runtime):
+    note("generic loop errors: wrong dat type, unknown loop")
+    cells = op2.op_decl_set(10, "cells")
+    pa = op2.op_decl_dat(cells, 3, "float", np.zeros(30, np.float32), "pa")
+    pb = op2.op_decl_dat(cells, 2, "float", np.zeros(20, np.float32), "pb")
+    s = np.zeros(2)
+    tot = np.zeros(2, np.float32)
+    imax = np.zeros(1, np.int32)
+    dmin = np.zeros(1)
+    with pytest.raises(op2.OP2Error, match="type does not match"):
+        op2.op_par_loop(None, "gen_direct", cells,
+                        op_arg_dat(pa, -1, OP_ID, 3, "float", OP_READ),
+                        op_arg_dat(pb, -1, OP_ID, 2, "float", OP_WRITE),
+                        op_arg_gbl(s, 2, "double", OP_READ), op_arg_gbl(tot, 2, "float", OP_INC),
+                        op_arg_gbl(imax, 1, "int", OP_MAX), op_arg_gbl(dmin, 1, "double", OP_MIN))
+    with pytest.raises(op2.OP2Error, match="no registered generic loop"):
+        op2.op_par_loop(None, "never_registered", cells,
+                        op_arg_dat(pb, -1, OP_ID, 2, "float", OP_WRITE))

@@ -226,3 +226,29 @@ def test_renumber_keeps_components_in_order(host_runtime):
     new_mesh = np.empty(b.union.ncell, np.int64)
     new_mesh[p] = b.cmesh
     assert np.all(np.diff(new_mesh) >= 0)
+
+
+def test_generic_plan_colours_every_indirect_write(host_runtime):
+    """The generic loop path's plan colours on OP_RW / OP_WRITE / OP_MIN /
+    OP_MAX as well as OP_INC, and takes more than four (map, index) pairs;
+    the exact verifier checks both colourings."""
+    mesh = M.o_mesh(40, 20, "random", seed=7)
+    d = declare(mesh)
+    ng = 37
+    groups = op2.op_decl_set(ng, "groups")
+    rng = np.random.Generator(np.random.PCG64(1))
+    cg = np.ascontiguousarray(rng.integers(0, ng, (mesh.ncell, 2)).astype(np.int32))
+    pcg = op2.op_decl_map(d["cells"], groups, 2, cg, "pcg")
+    v = op2.op_decl_dat_host(d["nodes"], 1, "double", np.zeros(mesh.nnode), "v")
+    g = op2.op_decl_dat_host(groups, 1, "int", np.zeros(ng, np.int32), "g")
+    c = op2.op_decl_dat_host(d["cells"], 1, "float", np.zeros(mesh.ncell, np.float32), "c")
+    args = [op2.op_arg_dat(c, -1, None, 1, "float", op2.OP_READ)]
+    args += [op2.op_arg_dat(v, k, d["pcell"], 1, "double", op2.OP_RW) for k in range(4)]
+    args += [op2.op_arg_dat(g, 0, pcg, 1, "int", op2.OP_MAX),
+             op2.op_arg_dat(g, 1, pcg, 1, "int", op2.OP_MIN)]
+    s = op2.plan_summary("gen_cell", d["cells"], args, op2.PLAN_GENERIC, 128)
+    assert s["verify_errors"] == 0
+    assert s["ncolors"] > 1 and s["mean_thread_colours"] > 1
+    # the same loop on the device kernels' strategies is refused (six pairs)
+    with pytest.raises(op2.OP2Error, match="at most 4"):
+        op2.plan_summary("gen_cell", d["cells"], args, op2.STRATEGY_COLOURED, 128)
