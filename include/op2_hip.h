@@ -240,6 +240,23 @@ int const *op2_map_device_ptr(op_map map);
  * call that asks for more). */
 void *op2_device_scratch(size_t bytes);
 
+/* Checkpoint and fast-forward restart (doc/dev.tex:1080-1087 describes
+ * op_checkpoint; the reference does not implement it).
+ * op2_checkpoint_record(path): from now on count loop calls and keep the
+ * values of OP_GBL OP_INC/OP_MIN/OP_MAX arguments after each loop (deferred
+ * reductions must be off).  op_checkpoint(): write every dat (in the user's
+ * numbering), the counts and the kept reduction values to `path`; returns 0.
+ * op2_checkpoint_restart(path), right after op_init in a new run of the
+ * same program: loop calls before that checkpoint are skipped and hand back
+ * their recorded reduction values; the op_checkpoint call that wrote the file
+ * restores every dat and returns 1, and recording continues.  Returns -1 on
+ * error. */
+int op2_checkpoint_record(char const *path);
+int op2_checkpoint_restart(char const *path);
+int op_checkpoint(void);
+/* 0 off, 1 recording, 2 fast-forwarding; loop calls and checkpoints so far */
+int op2_checkpoint_state(unsigned long long *loops, unsigned long long *calls);
+
 /* Device seconds, call count and algorithmic bytes per call of a loop
  * (device time from HIP events on the runtime stream).  Returns 0 if the
  * loop has run. */

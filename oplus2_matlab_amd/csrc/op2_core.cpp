@@ -250,6 +250,7 @@ static void clear_all() {
   r.set_perm.clear();
   r.set_segments.clear();
   r.map_internal.clear();
+  checkpoint_reset();
   for (auto &o : r.online)
     for (auto &ev : o.second.events)
       for (auto &e : ev) {

@@ -1465,8 +1465,21 @@ void online_call(const std::string &name, OnlineTune &ot, Loop l, int rb, op_set
 
 }  // namespace
 
+namespace {
+int run_loop(const char *name, op_set set, int nargs, op_arg *args);
+}
+
 int par_loop(const char *name, op_set set, int nargs, op_arg *args) {
   if (!name || !set || nargs <= 0 || !args) fail("op_par_loop error -- invalid arguments");
+  if (checkpoint_skip(name, nargs, args)) return 0;
+  const int rc = run_loop(name, set, nargs, args);
+  checkpoint_after(name, nargs, args);
+  return rc;
+}
+
+namespace {
+
+int run_loop(const char *name, op_set set, int nargs, op_arg *args) {
   ensure_device();
   if (OP_diags > 0) {
     int ninds = 0;
@@ -1543,6 +1556,8 @@ int par_loop(const char *name, op_set set, int nargs, op_arg *args) {
   timing_end(t, start);
   return 0;
 }
+
+}  // namespace
 
 // Autotuner: times candidate (strategy, part size) pairs of one loop on the
 // live mesh and keeps the fastest as the loop's configuration.  Every dat

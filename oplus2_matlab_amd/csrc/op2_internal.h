@@ -200,6 +200,13 @@ void h2d(void *dst, const void *src, size_t bytes);
 void d2h(void *dst, const void *src, size_t bytes);
 void ensure_map_device(op_map m);
 
+// checkpoint / fast-forward restart (op2_checkpoint.cpp): skip a loop call
+// before a restart's checkpoint (handing back its recorded reductions);
+// count / record a loop call that ran; forget everything (op_exit)
+bool checkpoint_skip(const char *name, int nargs, const op_arg *args);
+void checkpoint_after(const char *name, int nargs, const op_arg *args);
+void checkpoint_reset();
+
 // renumbering (op2_renumber.cpp)
 inline const int *map_host(op_map m) { return m->map_i ? m->map_i : m->map; }
 const std::vector<int> *set_new_of_old(op_set s);

@@ -105,6 +105,10 @@ SIGNATURES = {
     "op2_map_device_ptr": (_P, [_P]),
     "op2_device_scratch": (_P, [C.c_size_t]),
     "op2_report_error": (C.c_int, [C.c_char_p]),
+    "op2_checkpoint_record": (C.c_int, [C.c_char_p]),
+    "op2_checkpoint_restart": (C.c_int, [C.c_char_p]),
+    "op_checkpoint": (C.c_int, []),
+    "op2_checkpoint_state": (C.c_int, [C.POINTER(C.c_ulonglong), C.POINTER(C.c_ulonglong)]),
     "op2_renumber": (C.c_int, [_P]),
     "op2_renumber_export": (C.c_int, [_P, _P]),
     "op2_autotune_online": (C.c_int, [C.c_char_p, C.c_int]),
@@ -342,6 +346,34 @@ def autotune(name: str, set_: Set, args, reps: int = 5) -> dict:
              C.byref(part), C.byref(us)) != 0:
         raise OP2Error(lib().op2_last_error().decode())
     return {"strategy": st.value, "part_size": part.value, "us": us.value}
+
+
+def checkpoint_record(path: str) -> None:
+    """Start recording for op_checkpoint (loop counts and reduction values)."""
+    if _call("op2_checkpoint_record", _b(path)) != 0:
+        raise OP2Error(lib().op2_last_error().decode())
+
+
+def checkpoint_restart(path: str) -> None:
+    """Fast-forward this run to the checkpoint stored in `path`."""
+    if _call("op2_checkpoint_restart", _b(path)) != 0:
+        raise OP2Error(lib().op2_last_error().decode())
+
+
+def op_checkpoint() -> bool:
+    """Writes a checkpoint; in a restart, returns True at the checkpoint
+    where the state was restored."""
+    rc = _call("op_checkpoint")
+    if rc < 0:
+        raise OP2Error(lib().op2_last_error().decode())
+    return rc == 1
+
+
+def checkpoint_state() -> tuple:
+    """(mode, loop calls, checkpoints): mode 0 off, 1 recording, 2 forwarding."""
+    loops, calls = C.c_ulonglong(), C.c_ulonglong()
+    mode = _call("op2_checkpoint_state", C.byref(loops), C.byref(calls))
+    return mode, loops.value, calls.value
 
 
 def renumber(base: "Map") -> None:

@@ -99,3 +99,34 @@ def test_online_autotune_arguments():
     op2.autotune_online("res_calc", 2)
     assert op2.autotune_online_result("res_calc") is None
     op2.op_exit()
+
+
+def test_checkpoint_round_trip_host_dats(tmp_path):
+    """op_checkpoint writes every dat and a restarted run gets them back at
+    the same checkpoint (host-only dats: no GPU needed); a program that
+    declares other dats is refused."""
+    path = str(tmp_path / "ck.bin")
+    L = op2.lib()
+    L.op_init_core(0, None, 0)
+    cells = op2.op_decl_set(7, "cells")
+    a = np.arange(14, dtype=np.float64)
+    op2.op_decl_dat_host(cells, 2, "double", a, "a")
+    op2.checkpoint_record(path)
+    assert op2.op_checkpoint() is False
+    op2.op_exit()
+    L.op_init_core(0, None, 0)
+    cells = op2.op_decl_set(7, "cells")
+    b = np.zeros(14)
+    op2.op_decl_dat_host(cells, 2, "double", b, "a")
+    op2.checkpoint_restart(path)
+    assert op2.checkpoint_state()[0] == 2
+    assert op2.op_checkpoint() is True
+    assert np.array_equal(b, a) and op2.checkpoint_state()[0] == 1
+    op2.op_exit()
+    L.op_init_core(0, None, 0)
+    cells = op2.op_decl_set(7, "cells")
+    op2.op_decl_dat_host(cells, 1, "double", np.zeros(7), "a")
+    op2.checkpoint_restart(path)
+    with pytest.raises(op2.OP2Error, match="does not match"):
+        op2.op_checkpoint()
+    op2.op_exit()
