@@ -27,11 +27,19 @@ constexpr int NT = 256;  // threads per workgroup (4 waves)
 // ============================================================================
 
 // save_soln: qold = q, as 16-byte chunks
+typedef unsigned op2_v4u __attribute__((ext_vector_type(4)));
+
+// save_soln: 16-byte copy.  The copy (qold) is next read by update, after
+// adt_calc and res_calc: non-temporal stores keep it from displacing their
+// data in the caches.
 __global__ __launch_bounds__(NT) void k_copy16(const uint4 *__restrict__ src,
                                                uint4 *__restrict__ dst, long long n) {
   long long i = (long long)blockIdx.x * NT + threadIdx.x;
   const long long stride = (long long)gridDim.x * NT;
-  for (; i < n; i += stride) dst[i] = src[i];
+  for (; i < n; i += stride) {
+    const op2_v4u v = *reinterpret_cast<const op2_v4u *>(src + i);
+    __builtin_nontemporal_store(v, reinterpret_cast<op2_v4u *>(dst + i));
+  }
 }
 
 // adt_calc: one cell per thread, nodes gathered through pcell
@@ -448,6 +456,30 @@ __global__ __launch_bounds__(NT) void k_res_calc_direct(const int2 *__restrict__
   for (int d = 0; d < 4; d++) atomicAdd(res + 4 * (size_t)c.x + d, f[d]);
 #pragma unroll
   for (int d = 0; d < 4; d++) atomicAdd(res + 4 * (size_t)c.y + d, -f[d]);
+}
+
+// Boundary-edge loop without a plan (bres_calc): one thread per boundary
+// edge, global atomics.  The loop is small (one edge per boundary cell on
+// the O-mesh), so its time is latency: a single pass of dependent loads.
+// qinf_dat / bmap: the batched form's free stream of the edge's mesh.
+template <typename T>
+__global__ __launch_bounds__(NT) void k_bres_calc_direct(
+    const int2 *__restrict__ pbedge, const int *__restrict__ pbecell, const T *__restrict__ x,
+    const T *__restrict__ q, const T *__restrict__ adt, T *__restrict__ res,
+    const int *__restrict__ bound, const T *__restrict__ qinf_dat, const int *__restrict__ bmap,
+    int nbedge, AirfoilConsts<T> k) {
+  const int e = blockIdx.x * NT + threadIdx.x;
+  if (e >= nbedge) return;
+  const int2 n = pbedge[e];
+  const int c = pbecell[e];
+  T x1[2], x2[2], q1[4], f[4];
+  Vec<T>::load2(x + 2 * (size_t)n.x, x1);
+  Vec<T>::load2(x + 2 * (size_t)n.y, x2);
+  Vec<T>::load4(q + 4 * (size_t)c, q1);
+  const T *qi = qinf_dat ? qinf_dat + 4 * (size_t)bmap[e] : k.qinf;
+  boundary_flux(x1, x2, q1, adt[c], bound[e], qi, k, f);
+#pragma unroll
+  for (int d = 0; d < 4; d++) atomicAdd(res + 4 * (size_t)c + d, f[d]);
 }
 
 // Staged boundary-edge loop (bres_calc): as k_res_calc with one increment per
@@ -1064,10 +1096,20 @@ int narrow_strategy(int s) {
 
 template <typename T>
 void run_bres_calc(const char *name, op_set set, int nargs, op_arg *a) {
-  const int strategy = narrow_strategy(strategy_of(name, sizeof(T)));
+  const int requested = strategy_of(name, sizeof(T));
+  const int strategy = narrow_strategy(requested);
   // batched form: the free stream of each boundary edge's mesh, through a map
   const T *qd = nargs == 7 ? (const T *)a[6].dat->data_d : nullptr;
   const int *bm = nargs == 7 ? a[6].map->map_d : nullptr;
+  if (requested == OP2_STRATEGY_DIRECT) {
+    if (set->size == 0) return;
+    hipLaunchKernelGGL(k_bres_calc_direct<T>, dim3((set->size + NT - 1) / NT), dim3(NT), 0,
+                       rt().stream, (const int2 *)a[0].map->map_d, (const int *)a[2].map->map_d,
+                       (const T *)a[0].dat->data_d, (const T *)a[2].dat->data_d,
+                       (const T *)a[3].dat->data_d, (T *)a[4].dat->data_d,
+                       (const int *)a[5].dat->data_d, qd, bm, set->size, airfoil_consts<T>());
+    return;
+  }
   if (strategy == OP2_STRATEGY_OWNER) {
     Plan *p = plan_for(name, set, nargs, a, strategy, part_of(name, NT));
     const PlanGroup &nodes = p->groups[p->slots[0].first];
@@ -1353,6 +1395,7 @@ std::vector<LoopConfig> tune_candidates(Loop l) {
     for (int part : {128, 256}) c.push_back({OP2_STRATEGY_OWNER, part, 0});
   } else {
     for (int part : {128, 256, 512}) c.push_back({OP2_STRATEGY_OWNER, part, 0});
+    if (l == Loop::kBresCalc) c.push_back({OP2_STRATEGY_DIRECT, 0, 0});
   }
   c.push_back({OP2_STRATEGY_ATOMIC, 256, 0});
   for (int part : {128, 256}) c.push_back({OP2_STRATEGY_COLOURED, part, 0});

@@ -46,7 +46,7 @@ def check_meshes(batch, hist, q, niter, form, tol, which=None):
         assert eh < tol and eq < tol, (i, eh, eq)
 
 
-STRATS = [op2.STRATEGY_ATOMIC, op2.STRATEGY_COLOURED, op2.STRATEGY_OWNER,
+STRATS = [op2.STRATEGY_ATOMIC, op2.STRATEGY_COLOURED, op2.STRATEGY_DIRECT, op2.STRATEGY_OWNER,
           op2.STRATEGY_OWNER_GATHER]
 
 
