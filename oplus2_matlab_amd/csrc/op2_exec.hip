@@ -42,15 +42,25 @@ __global__ __launch_bounds__(NT) void k_copy16(const uint4 *__restrict__ src,
   }
 }
 
-// adt_calc: one cell per thread, nodes gathered through pcell
+// adt_calc: one cell per thread, nodes gathered through pcell (measured:
+// the non-temporal map load pays in FP64 only)
 template <typename T>
 __global__ __launch_bounds__(NT) void k_adt_calc(const int4 *__restrict__ pcell,
                                                  const T *__restrict__ x,
                                                  const T *__restrict__ q, T *__restrict__ adt,
                                                  int ncell, AirfoilConsts<T> k) {
-  const int c = blockIdx.x * NT + threadIdx.x;
+  // each XCD takes a contiguous range of cells, so the nodes that
+  // neighbouring cells share are fetched into one L2
+  const int c = xcd_block(blockIdx.x, gridDim.x) * NT + threadIdx.x;
   if (c >= ncell) return;
-  const int4 nd = pcell[c];
+  int4 nd;
+  if (sizeof(T) == 8) {  // FP64: the map is read once, keep it out of the caches
+    typedef int v4 __attribute__((ext_vector_type(4)));
+    const v4 v = __builtin_nontemporal_load(reinterpret_cast<const v4 *>(pcell + c));
+    nd = make_int4(v.x, v.y, v.z, v.w);
+  } else {
+    nd = pcell[c];
+  }
   T x1[2], x2[2], x3[2], x4[2], qc[4];
   Vec<T>::load2(x + 2 * (size_t)nd.x, x1);
   Vec<T>::load2(x + 2 * (size_t)nd.y, x2);

@@ -356,7 +356,7 @@ def test_online_autotuner_runs_real_iterations(runtime):
     hist += [a.iterate() for _ in range(29)]
     r, b = op2.autotune_online_result("res_calc"), op2.autotune_online_result("bres_calc")
     note(f"online choices: res_calc {r}, bres_calc {b}")
-    assert r["strategy"] in (1, 2, 4, 5) and b["strategy"] in (1, 2, 4)
+    assert r["strategy"] in (1, 2, 4, 5) and b["strategy"] in (1, 2, 3, 4)
     _, oh = O.airfoil_seq(mesh, c, 30, "f64")
     assert np.max(np.abs(np.array(hist) - oh) / oh) < TOL64
 
