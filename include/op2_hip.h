@@ -214,7 +214,8 @@ typedef void (*op2_const_copy_fn)(void const *data, size_t bytes, void *stream);
 int op2_register_const(char const *name, op2_const_copy_fn fn);
 
 /* Execution arrays of the loop's generic plan (built and verified on first
- * use): contiguous blocks of `part_size` elements (0 = 256) whose elements
+ * use): contiguous blocks of `part_size` elements (0 = the loop's configured
+ * part size, else 256) whose elements
  * are sorted by thread colour, thread colours and block colours such that
  * no two elements running together write the same indirect target (every
  * access other than OP_READ counts).  Pointers are device arrays except

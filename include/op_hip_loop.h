@@ -85,20 +85,31 @@ struct Local {
   elem_t<T> v[kGblBytes / sizeof(elem_t<T>)];
 };
 
+// Argument classes known at compile time: with G >= 0 the last G arguments
+// are OP_GBL and the others dats (the usual order; G up to 4), so dat
+// pointers stay global and the OP_GBL copies can live in registers; G < 0:
+// any order, decided per argument at run time (flat pointers, copies in
+// private memory).
+template <int I, int N, int G>
+struct ArgClass {
+  static constexpr bool dat = G >= 0 && I < N - G;  // certainly a dat
+  static constexpr bool gbl = G >= 0 && I >= N - G; // certainly OP_GBL
+};
+
+
 template <class T>
 __device__ __forceinline__ void local_init(const Arg &d, Local<T> &l) {
   if (d.kind == kDat) return;
-  if (d.kind == kGblInc) {
-    for (int c = 0; c < d.dim; c++) l.v[c] = elem_t<T>(0);
-  } else {
-    const elem_t<T> *g = reinterpret_cast<const elem_t<T> *>(d.gbl);
-    for (int c = 0; c < d.dim; c++) l.v[c] = g[c];
-  }
+  constexpr int n = kGblBytes / sizeof(elem_t<T>);
+  const elem_t<T> *g = reinterpret_cast<const elem_t<T> *>(d.gbl);
+  // the whole copy, with constant indices (the gbl bytes past dim are zero)
+#pragma unroll
+  for (int c = 0; c < n; c++) l.v[c] = d.kind == kGblInc ? elem_t<T>(0) : g[c];
 }
 
-template <class T>
+template <class T, bool DAT, bool GBL>
 __device__ __forceinline__ T *arg_ptr(const Arg &d, int e, Local<T> &l) {
-  if (d.kind != kDat) return l.v;
+  if (GBL || (!DAT && d.kind != kDat)) return l.v;
   const int el = d.map ? d.map[(size_t)e * d.mapdim + d.idx] : e;
   return reinterpret_cast<T *>(d.data + (size_t)el * d.bytes);
 }
@@ -126,41 +137,49 @@ __device__ __forceinline__ double block_red(int kind, double v, double *lds) {
   return r;
 }
 
-template <class T>
+template <class T, bool DAT>
 __device__ __forceinline__ void reduce_arg(const Arg &d, const Local<T> &l, double *partials,
                                            int slot, int nred, double *lds) {
-  if (d.kind < kGblInc) return;
-  for (int c = 0; c < d.dim; c++) {
-    const double r = block_red(d.kind, (double)l.v[c], lds);
-    if (threadIdx.x == 0) partials[(size_t)slot * nred + d.red + c] = r;
+  if (DAT || d.kind < kGblInc) return;
+  constexpr int n = kGblBytes / sizeof(elem_t<T>);
+#pragma unroll
+  for (int c = 0; c < n; c++) {
+    if (c < d.dim) {  // uniform
+      const double r = block_red(d.kind, (double)l.v[c], lds);
+      if (threadIdx.x == 0) partials[(size_t)slot * nred + d.red + c] = r;
+    }
   }
 }
 
-template <auto F, class... T, size_t... I>
+template <auto F, int G, class... T, size_t... I>
 __device__ __forceinline__ void call(const Args<sizeof...(T)> &A, int e,
                                      std::tuple<Local<T>...> &loc, std::index_sequence<I...>) {
-  F(arg_ptr<T>(A.a[I], e, std::get<I>(loc))...);
+  constexpr int N = sizeof...(T);
+  F(arg_ptr<T, ArgClass<I, N, G>::dat, ArgClass<I, N, G>::gbl>(A.a[I], e, std::get<I>(loc))...);
 }
 
-template <class... T, size_t... I>
+template <int G, class... T, size_t... I>
 __device__ __forceinline__ void init_all(const Args<sizeof...(T)> &A,
                                          std::tuple<Local<T>...> &loc, std::index_sequence<I...>) {
-  (local_init<T>(A.a[I], std::get<I>(loc)), ...);
+  constexpr int N = sizeof...(T);
+  ((ArgClass<I, N, G>::dat ? (void)0 : local_init<T>(A.a[I], std::get<I>(loc))), ...);
 }
 
-template <class... T, size_t... I>
+template <int G, class... T, size_t... I>
 __device__ __forceinline__ void reduce_all(const Args<sizeof...(T)> &A,
                                            std::tuple<Local<T>...> &loc, double *partials,
                                            int slot, int nred, double *lds,
                                            std::index_sequence<I...>) {
-  (reduce_arg<T>(A.a[I], std::get<I>(loc), partials, slot, nred, lds), ...);
+  constexpr int N = sizeof...(T);
+  (reduce_arg<T, ArgClass<I, N, G>::dat>(A.a[I], std::get<I>(loc), partials, slot, nred, lds),
+   ...);
 }
 
 // One pass of the loop.  With a plan (blk_off != null) the workgroup runs
 // plan block blkmap[blk_first + blockIdx.x] one thread colour at a time;
 // without, a grid-stride pass over the set.  Reduction partials go to
 // partials[slot * nred + component], slot = plan block or workgroup.
-template <auto F, class... T>
+template <auto F, int G, class... T>
 __global__ __launch_bounds__(kThreads) void k_loop(Args<sizeof...(T)> A, int n, int nred,
                                                    int const *blk_off, int const *perm,
                                                    int const *col_off, int const *col_base,
@@ -169,7 +188,7 @@ __global__ __launch_bounds__(kThreads) void k_loop(Args<sizeof...(T)> A, int n, 
   __shared__ double lds[kThreads / 64];
   constexpr auto seq = std::index_sequence_for<T...>{};
   std::tuple<Local<T>...> loc;
-  init_all<T...>(A, loc, seq);
+  init_all<G, T...>(A, loc, seq);
   int slot;
   if (blk_off) {
     const int b = blkmap[blk_first + blockIdx.x];
@@ -178,15 +197,15 @@ __global__ __launch_bounds__(kThreads) void k_loop(Args<sizeof...(T)> A, int n, 
     for (int c = 0; c < ncol; c++) {
       const int hi = col_off[cb + c + 1];
       for (int p = col_off[cb + c] + (int)threadIdx.x; p < hi; p += kThreads)
-        call<F, T...>(A, perm ? perm[p] : p, loc, seq);
+        call<F, G, T...>(A, perm ? perm[p] : p, loc, seq);
       __syncthreads();  // the next colour sees this colour's writes
     }
   } else {
     slot = blockIdx.x;
     for (int e = blockIdx.x * kThreads + threadIdx.x; e < n; e += gridDim.x * kThreads)
-      call<F, T...>(A, e, loc, seq);
+      call<F, G, T...>(A, e, loc, seq);
   }
-  if (nred > 0) reduce_all<T...>(A, loc, partials, slot, nred, lds, seq);
+  if (nred > 0) reduce_all<G, T...>(A, loc, partials, slot, nred, lds, seq);
 }
 
 // Folds the partials of every reduction component (one workgroup each).
@@ -291,6 +310,18 @@ int run(char const *name, op_set set, int nargs, op_arg *args, std::tuple<T...> 
   bool plan = false;
   const int rc = fill_all<T...>(name, args, A, rk, plan, seq);
   if (rc) return rc;
+  // the kernel specialised for the argument classes: G trailing OP_GBL
+  // arguments (0-3) and dats before them, or the run-time form
+  int G = 0;
+  while (G < N && args[N - 1 - G].argtype == OP_ARG_GBL) G++;
+  bool trailing = G <= 4;
+  for (int i = 0; i < N - G; i++) trailing = trailing && args[i].argtype == OP_ARG_DAT;
+  auto kern = !trailing ? k_loop<F, -1, T...>
+              : G == 0  ? k_loop<F, 0, T...>
+              : G == 1  ? k_loop<F, (N >= 1 ? 1 : 0), T...>
+              : G == 2  ? k_loop<F, (N >= 2 ? 2 : 0), T...>
+              : G == 3  ? k_loop<F, (N >= 3 ? 3 : 0), T...>
+                        : k_loop<F, (N >= 4 ? 4 : 0), T...>;
   hipStream_t st = (hipStream_t)op2_stream();
   const int n = set->size;
   int nslots = 0;
@@ -308,7 +339,7 @@ int run(char const *name, op_set set, int nargs, op_arg *args, std::tuple<T...> 
     for (int c = 0; c < gp.ncolors; c++) {
       const int first = gp.colblk_off[c], nb = gp.colblk_off[c + 1] - first;
       if (nb == 0) continue;
-      hipLaunchKernelGGL((k_loop<F, T...>), dim3(nb), dim3(kThreads), 0, st, A, n, rk.n,
+      hipLaunchKernelGGL(kern, dim3(nb), dim3(kThreads), 0, st, A, n, rk.n,
                          gp.blk_off, gp.perm, gp.col_off, gp.col_base, gp.blk_ncol, gp.blkmap,
                          first, partials);
     }
@@ -316,7 +347,7 @@ int run(char const *name, op_set set, int nargs, op_arg *args, std::tuple<T...> 
     int grid = (n + kThreads - 1) / kThreads;
     grid = grid < 1 ? 1 : grid > kDirectGrid ? kDirectGrid : grid;
     if (scratch(grid)) return -1;
-    hipLaunchKernelGGL((k_loop<F, T...>), dim3(grid), dim3(kThreads), 0, st, A, n, rk.n,
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(kThreads), 0, st, A, n, rk.n,
                        (int const *)nullptr, (int const *)nullptr, (int const *)nullptr,
                        (int const *)nullptr, (int const *)nullptr, (int const *)nullptr, 0,
                        partials);

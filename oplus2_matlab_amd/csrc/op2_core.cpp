@@ -791,7 +791,10 @@ int op2_generic_plan_get(char const *name, op_set set, int nargs, op_arg *args, 
                          op2_generic_plan *out) {
   API_BEGIN
   if (!name || !set || nargs <= 0 || !args || !out) fail("op2_generic_plan_get: invalid arguments");
-  PlanRequest req{name, set, nargs, args, kStrategyGeneric, part_size > 0 ? part_size : 256};
+  // block size: the argument, else the loop's configured part size, else 256
+  int part = part_size > 0 ? part_size : config_for(name).part_size;
+  if (part <= 0) part = 256;
+  PlanRequest req{name, set, nargs, args, kStrategyGeneric, part};
   Plan *p = get_plan(req);
   p->upload();
   out->nblocks = p->nblocks;

@@ -49,3 +49,53 @@ __device__ inline void gen_cell(const float *c, double *v0, double *v1, double *
 OP2_DEVICE_LOOP(gen_direct)
 OP2_DEVICE_LOOP(gen_edge)
 OP2_DEVICE_LOOP(gen_cell)
+
+// an indirect loop with global reductions: total and longest edge length,
+// edges counted per node (OP_INC through the map)
+__device__ inline void gen_edge_sum(const double *x1, const double *x2, int *c1, int *c2,
+                                    double *total, double *longest) {
+  const double dx = x2[0] - x1[0], dy = x2[1] - x1[1];
+  const double l = sqrt(dx * dx + dy * dy);
+  *total += l;
+  *longest = MAX(*longest, l);
+  *c1 += 1;
+  *c2 += 1;
+}
+
+// airfoil/res_calc.h's arithmetic as a generic loop (8 arguments, 2 maps),
+// with its own device constants: the generic path against the hand-written
+// res_calc kernels
+static __constant__ double gm1;
+static __constant__ double eps;
+OP2_DEVICE_CONST(gm1)
+OP2_DEVICE_CONST(eps)
+
+__device__ inline void gen_res_calc(const double *x1, const double *x2, const double *q1,
+                                    const double *q2, const double *adt1, const double *adt2,
+                                    double *res1, double *res2) {
+  double dx, dy, mu, ri, p1, vol1, p2, vol2, f;
+  dx = x1[0] - x2[0];
+  dy = x1[1] - x2[1];
+  ri = 1.0 / q1[0];
+  p1 = gm1 * (q1[3] - 0.5 * ri * (q1[1] * q1[1] + q1[2] * q1[2]));
+  vol1 = ri * (q1[1] * dy - q1[2] * dx);
+  ri = 1.0 / q2[0];
+  p2 = gm1 * (q2[3] - 0.5 * ri * (q2[1] * q2[1] + q2[2] * q2[2]));
+  vol2 = ri * (q2[1] * dy - q2[2] * dx);
+  mu = 0.5 * ((*adt1) + (*adt2)) * eps;
+  f = 0.5 * (vol1 * q1[0] + vol2 * q2[0]) + mu * (q1[0] - q2[0]);
+  res1[0] += f;
+  res2[0] -= f;
+  f = 0.5 * (vol1 * q1[1] + p1 * dy + vol2 * q2[1] + p2 * dy) + mu * (q1[1] - q2[1]);
+  res1[1] += f;
+  res2[1] -= f;
+  f = 0.5 * (vol1 * q1[2] - p1 * dx + vol2 * q2[2] - p2 * dx) + mu * (q1[2] - q2[2]);
+  res1[2] += f;
+  res2[2] -= f;
+  f = 0.5 * (vol1 * (q1[3] + p1) + vol2 * (q2[3] + p2)) + mu * (q1[3] - q2[3]);
+  res1[3] += f;
+  res2[3] -= f;
+}
+
+OP2_DEVICE_LOOP(gen_edge_sum)
+OP2_DEVICE_LOOP(gen_res_calc)

@@ -182,3 +182,54 @@ def test_generic_errors(runtime):
     with pytest.raises(op2.OP2Error, match="no registered generic loop"):
         op2.op_par_loop(None, "never_registered", cells,
                         op_arg_dat(pb, -1, OP_ID, 2, "float", OP_WRITE))
+
+
+def test_reductions_in_an_indirect_loop(runtime):
+    """OP_GBL INC / MAX inside a coloured (indirect-increment) loop."""
+    note("generic edge loop with OP_GBL INC and MAX and int increments")
+    mesh = M.o_mesh(64, 32, "random", seed=9)
+    nodes = op2.op_decl_set(mesh.nnode, "nodes")
+    edges = op2.op_decl_set(mesh.nedge, "edges")
+    pedge = op2.op_decl_map(edges, nodes, 2, mesh.edge, "pedge")
+    px = op2.op_decl_dat(nodes, 2, "double", np.ascontiguousarray(mesh.x), "px")
+    cnt = np.zeros(mesh.nnode, np.int32)
+    pc = op2.op_decl_dat(nodes, 1, "int", cnt, "pc")
+    total = np.array([1.0])
+    longest = np.array([0.0])
+    op2.op_par_loop(None, "gen_edge_sum", edges,
+                    op_arg_dat(px, 0, pedge, 2, "double", OP_READ),
+                    op_arg_dat(px, 1, pedge, 2, "double", OP_READ),
+                    op_arg_dat(pc, 0, pedge, 1, "int", OP_INC),
+                    op_arg_dat(pc, 1, pedge, 1, "int", OP_INC),
+                    op_arg_gbl(total, 1, "double", OP_INC),
+                    op_arg_gbl(longest, 1, "double", OP_MAX))
+    e1, e2 = mesh.edge[:, 0], mesh.edge[:, 1]
+    ln = np.hypot(mesh.x[e2, 0] - mesh.x[e1, 0], mesh.x[e2, 1] - mesh.x[e1, 1])
+    want_cnt = np.bincount(e1, minlength=mesh.nnode) + np.bincount(e2, minlength=mesh.nnode)
+    assert np.array_equal(op2.op_fetch_data(pc), want_cnt.astype(np.int32))
+    np.testing.assert_allclose(total[0], 1.0 + ln.sum(), rtol=1e-13)
+    np.testing.assert_allclose(longest[0], ln.max(), rtol=1e-15)
+
+
+@pytest.mark.parametrize("renumber", [False, True])
+def test_generic_res_calc_equals_the_device_kernel(runtime, renumber):
+    """airfoil/res_calc.h's arithmetic through the generic path gives the
+    hand-written res_calc kernel's increments (to summation order)."""
+    note(f"generic res_calc vs the device kernel, 120x60 random, renumber={renumber}")
+    from oplus2_matlab_amd import airfoil as AF
+    mesh = M.o_mesh(120, 60, "random", seed=4)
+    c = M.flow_constants().as_array()
+    rng = np.random.Generator(np.random.PCG64(12))
+    q0 = np.tile(c[6:10], (mesh.ncell, 1)) * (1 + 0.02 * rng.standard_normal((mesh.ncell, 4)))
+    a = AF.Airfoil(mesh, c, "double", q0=q0, renumber=renumber)
+    a.adt_calc()
+    a.res_calc()
+    native = op2.op_fetch_data(a.p_res).reshape(-1, 4).copy()
+    res2 = np.zeros(4 * mesh.ncell)
+    pres2 = op2.op_decl_dat(a.cells, 4, "double", res2, "pres2")
+    args = list(a.res_calc_args())
+    args[6] = op_arg_dat(pres2, 0, a.pecell, 4, "double", OP_INC)
+    args[7] = op_arg_dat(pres2, 1, a.pecell, 4, "double", OP_INC)
+    op2.op_par_loop(None, "gen_res_calc", a.edges, *args)
+    got = op2.op_fetch_data(pres2).reshape(-1, 4)
+    assert np.max(np.abs(got - native)) <= 1e-13 * np.max(np.abs(native))
