@@ -181,7 +181,7 @@ def measure_batch(a):
             "tuned": tuned, "loops": loops}
 
 
-def accuracy(mesh, consts, iters):
+def accuracy(mesh, consts, iters, renumber):
     """FP32 form against FP64 after `iters` iterations (graph replay)."""
     import numpy as np
 
@@ -192,7 +192,8 @@ def accuracy(mesh, consts, iters):
     for dtype in ("double", "float"):
         op2.op_init([], 0)
         try:
-            out[dtype] = AF.run_airfoil(mesh, consts, iters, dtype, graph=True)
+            out[dtype] = AF.run_airfoil(mesh, consts, iters, dtype, graph=True,
+                                        renumber=renumber)
         finally:
             op2.op_exit()
     h64, q64 = out["double"][0], out["double"][1]
@@ -283,7 +284,8 @@ def main():
                                "tuned": r32["tuned"],
                                "loops": r32["loops"]}
         if a.accuracy_iters > 0:
-            result["fp32_form"]["accuracy"] = accuracy(mesh, consts, a.accuracy_iters)
+            result["fp32_form"]["accuracy"] = accuracy(mesh, consts, a.accuracy_iters,
+                                                       a.renumber == "on")
     if a.batch_meshes > 0:
         rb = measure_batch(a)
         rb["ratio_vs_single_mesh"] = rb["value"] / r64["value"]

@@ -28,12 +28,5 @@ void op_par_loop(void (*kernel)(T *...), char const *name, op_set set, A... args
   op2_par_loop(name, set, (int)sizeof...(A), arr);
 }
 
-#define OP2_STUB(NAME)                                                       \
-  template <class... A>                                                      \
-  inline void op_par_loop_##NAME(char const *name, op_set set, A... args) {   \
-    op_arg arr[sizeof...(A)] = {args...};                                    \
-    op2_par_loop(name, set, (int)sizeof...(A), arr);                         \
-  }
-
 #endif /* __cplusplus */
 #endif

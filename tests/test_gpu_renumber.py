@@ -126,3 +126,17 @@ def test_declarations_after_renumbering(runtime):
     oadt = np.zeros(mesh.ncell)
     O.adt_calc("f64", mesh, mesh.x.copy(), q0.copy(), oadt, c)
     assert relmax(op2.op_fetch_data(a.p_adt), oadt) < 1e-15
+
+
+def test_renumbered_fp32_form(runtime):
+    """The FP32 form on renumbered sets against the oracle's FP32 form."""
+    note("renumbered airfoil fp32 form 48x24 random, 100 iterations (graph)")
+    configure(op2.STRATEGY_OWNER_GATHER, 256)
+    mesh = M.o_mesh(48, 24, "random", seed=5)
+    c = M.flow_constants().as_array()
+    hist, q, _, _ = AF.run_airfoil(mesh, c, 100, "float", graph=True, renumber=True)
+    _, hmix = O.airfoil_seq(mesh, c, 100, "mix")
+    st64, h64 = O.airfoil_seq(mesh, c, 100, "f64")
+    assert np.max(np.abs(hist - hmix) / np.abs(hmix)) < 1e-4
+    assert np.max(np.abs(hist - h64) / np.abs(h64)) < 1e-4
+    assert np.max(np.abs(q - st64.q)) < 1e-4
