@@ -20,10 +20,15 @@ double): its throughput, speed-up over FP64, and its error against the FP64
 run after --accuracy-iters iterations (rms history relative error and
 max |q - q64|).
 
+orderings: the same 2.88M-cell mesh with every set numbered by a seed-random
+permutation, once renumbered by the runtime (op2_renumber) and once as
+given; cell-updates/s and per-loop GB/s of each (50 graph steps).
+
 batched: 64 meshes of 10k-400k cells with their own mach and alpha
 (7.12M cells) advanced as one union mesh, each boundary edge reading its
 mesh's free stream and update summing each mesh's rms; the iteration is a
-hipGraph as above.  value in cell-updates/s and its ratio to the single mesh.
+hipGraph as above.  value in cell-updates/s and its ratio to the single mesh,
+for FP64 and the FP32 form.
 
 cpu_baseline: the reference's OpenMP schedule (the oracle's bit-exact
 restatement of airfoil/<loop>_kernel.cpp on the reference plan) on this
@@ -63,6 +68,8 @@ def parse():
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-form measurement")
     ap.add_argument("--accuracy-iters", type=int, default=1000,
                     help="iterations of the FP32-vs-FP64 accuracy run (0 = skip)")
+    ap.add_argument("--no-orderings", dest="orderings", action="store_false",
+                    help="skip the random-ordering measurements")
     ap.add_argument("--batch-meshes", type=int, default=64,
                     help="meshes of the batched measurement (0 = skip)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
@@ -89,24 +96,32 @@ def measured_traffic(ni, nj, res_cfg):
     return None
 
 
-def measure(mesh, consts, dtype, a):
+def measure(mesh, consts, dtype, a, renumber=None, fixed=None, steps=None):
+    """renumber: override of --renumber; fixed: (res_calc strategy, part) instead of
+    --strategy; steps: override of --steps."""
     """Eager timed region (per-loop HIP events) then graph timed region."""
     from oplus2_matlab_amd import airfoil as AF
     from oplus2_matlab_amd import op2
 
+    ren = (a.renumber == "on") if renumber is None else renumber
+    nsteps = a.steps if steps is None else steps
     op2.op_init([], 0)
     try:
-        if STRATEGIES[a.strategy] >= 0 and (a.strategy != "auto" or a.part):
+        if fixed:
+            op2.set_loop_config("res_calc", fixed[0], fixed[1], 0)
+        elif STRATEGIES[a.strategy] >= 0 and (a.strategy != "auto" or a.part):
             op2.set_loop_config("res_calc", max(STRATEGIES[a.strategy], 0), a.part, 0)
-        af = AF.Airfoil(mesh, consts, dtype, renumber=a.renumber == "on")
-        tuned = (af.autotune(reps=3, online=True) if a.strategy == "tune" else
-                 af.autotune() if a.strategy == "tune-offline" else None)
+        af = AF.Airfoil(mesh, consts, dtype, renumber=ren)
+        tuned = None
+        if not fixed:
+            tuned = (af.autotune(reps=3, online=True) if a.strategy == "tune" else
+                     af.autotune() if a.strategy == "tune-offline" else None)
         for _ in range(a.warmup):
             af.iterate()
         op2.sync()
         op2.timing_reset()
         t0 = time.perf_counter()
-        for _ in range(a.steps):
+        for _ in range(nsteps):
             af.iterate()
         op2.sync()
         dt_eager = time.perf_counter() - t0
@@ -121,7 +136,7 @@ def measure(mesh, consts, dtype, a):
         af.run_graph(gid, a.warmup)
         op2.sync()
         t0 = time.perf_counter()
-        op2.graph_launch(gid, a.steps)
+        op2.graph_launch(gid, nsteps)
         op2.sync()
         dt_graph = time.perf_counter() - t0
         log = op2.reduction_log()
@@ -129,25 +144,24 @@ def measure(mesh, consts, dtype, a):
         op2.set_deferred_reductions(False)
     finally:
         op2.op_exit()
-    return {"value": mesh.ncell * a.steps / dt_graph, "ms_per_step": 1e3 * dt_graph / a.steps,
-            "eager_ms_per_step": 1e3 * dt_eager / a.steps, "loops": loops, "rms_last": rms_last,
+        op2.set_loop_config(None, 0, 0, 0)
+    return {"value": mesh.ncell * nsteps / dt_graph, "ms_per_step": 1e3 * dt_graph / nsteps,
+            "eager_ms_per_step": 1e3 * dt_eager / nsteps, "loops": loops, "rms_last": rms_last,
             "tuned": tuned}
 
 
-def measure_batch(a):
+def measure_batch(a, batch, dtype):
     """64 meshes (10k-400k cells, own mach and alpha) advanced together: one
     graph-captured iteration of the union mesh, replayed; per-mesh rms in the
     device log."""
     from oplus2_matlab_amd import airfoil as AF
-    from oplus2_matlab_amd import mesh as M
     from oplus2_matlab_amd import op2
 
-    batch = M.mesh_batch(nmesh=a.batch_meshes, seed=1)
     op2.op_init([], 0)
     try:
         if STRATEGIES[a.strategy] >= 0 and (a.strategy != "auto" or a.part):
             op2.set_loop_config("res_calc", max(STRATEGIES[a.strategy], 0), a.part, 0)
-        af = AF.AirfoilBatch(batch, "double", renumber=a.renumber == "on")
+        af = AF.AirfoilBatch(batch, dtype, renumber=a.renumber == "on")
         tuned = (af.autotune(reps=3, online=True) if a.strategy == "tune" else
                  af.autotune() if a.strategy == "tune-offline" else None)
         for _ in range(a.warmup):
@@ -171,6 +185,7 @@ def measure_batch(a):
         op2.set_deferred_reductions(False)
     finally:
         op2.op_exit()
+        op2.set_loop_config(None, 0, 0, 0)
     ncell = batch.union.ncell
     return {"value": ncell * a.steps / dt, "unit": "cell-updates/s",
             "ms_per_step": 1e3 * dt / a.steps, "meshes": batch.nmesh, "cells": ncell,
@@ -286,9 +301,31 @@ def main():
         if a.accuracy_iters > 0:
             result["fp32_form"]["accuracy"] = accuracy(mesh, consts, a.accuracy_iters,
                                                        a.renumber == "on")
+    if a.orderings:
+        # the same mesh with a seed-random numbering of every set: renumbered
+        # by the runtime, and as given (fixed res_calc strategies: the
+        # fastest of each in sweeps, to keep the run short)
+        mr = M.o_mesh(ni, nj, "random", seed=1)
+        orders = {}
+        for label, ren, fixed in (("random_renumbered", True, (5, 256)),
+                                  ("random_as_given", False, (4, 128))):
+            r = measure(mr, consts, "double", a, renumber=ren, fixed=fixed, steps=50)
+            orders[label] = {"value": r["value"], "unit": "cell-updates/s",
+                             "ms_per_step": r["ms_per_step"], "res_calc_strategy": fixed,
+                             "loops": {k: {"us_per_call": v["us_per_call"], "GBs": v["GBs"]}
+                                       for k, v in r["loops"].items()}}
+        result["orderings"] = orders
     if a.batch_meshes > 0:
-        rb = measure_batch(a)
+        batch = M.mesh_batch(nmesh=a.batch_meshes, seed=1)
+        rb = measure_batch(a, batch, "double")
         rb["ratio_vs_single_mesh"] = rb["value"] / r64["value"]
+        if not a.no_fp32:
+            rb32 = measure_batch(a, batch, "float")
+            rb["fp32_form"] = {"value": rb32["value"], "ms_per_step": rb32["ms_per_step"],
+                               "speedup_vs_fp64": rb32["value"] / rb["value"],
+                               "tuned": rb32["tuned"], "loops": rb32["loops"]}
+            if "fp32_form" in result:
+                rb["fp32_form"]["ratio_vs_single_mesh"] = rb32["value"] / result["fp32_form"]["value"]
         result["batched"] = rb
     result["cpu_baseline"] = cpu_baseline(mesh, consts, a.cpu_seconds) if a.cpu_seconds > 0 else None
     print(json.dumps(result))
