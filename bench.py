@@ -20,6 +20,9 @@ double): its throughput, speed-up over FP64, and its error against the FP64
 run after --accuracy-iters iterations (rms history relative error and
 max |q - q64|).
 
+mesh_600x300: the 180k-cell O-mesh with seed-random numbering (renumbered
+per --renumber), 200 graph steps.
+
 orderings: the same 2.88M-cell mesh with every set numbered by a seed-random
 permutation, once renumbered by the runtime (op2_renumber) and once as
 given; cell-updates/s and per-loop GB/s of each (50 graph steps).
@@ -70,6 +73,8 @@ def parse():
                     help="iterations of the FP32-vs-FP64 accuracy run (0 = skip)")
     ap.add_argument("--no-orderings", dest="orderings", action="store_false",
                     help="skip the random-ordering measurements")
+    ap.add_argument("--no-small-mesh", dest="small_mesh", action="store_false",
+                    help="skip the 600x300 (180k-cell) measurement")
     ap.add_argument("--batch-meshes", type=int, default=64,
                     help="meshes of the batched measurement (0 = skip)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
@@ -301,6 +306,16 @@ def main():
         if a.accuracy_iters > 0:
             result["fp32_form"]["accuracy"] = accuracy(mesh, consts, a.accuracy_iters,
                                                        a.renumber == "on")
+    if a.small_mesh:
+        # the 600 x 300 O-mesh (180k cells) with seed-random numbering
+        ms = M.o_mesh(600, 300, "random", seed=1)
+        r = measure(ms, consts, "double", a, renumber=a.renumber == "on", fixed=(5, 256), steps=200)
+        result["mesh_600x300"] = {
+            "value": r["value"], "unit": "cell-updates/s", "ms_per_step": r["ms_per_step"],
+            "cells": ms.ncell, "edges": ms.nedge, "ordering": "random",
+            "renumbered": a.renumber == "on",
+            "loops": {k: {"us_per_call": v["us_per_call"], "GBs": v["GBs"]}
+                      for k, v in r["loops"].items()}}
     if a.orderings:
         # the same mesh with a seed-random numbering of every set: renumbered
         # by the runtime, and as given (fixed res_calc strategies: the
