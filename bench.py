@@ -249,7 +249,7 @@ def main():
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if a.gpus != 1 or world != 1:
         print(f"bench.py runs on one GPU only (asked for {a.gpus}); multi-GPU sharding of "
-              "the mesh is a design note, not part of this build", file=sys.stderr)
+              "the mesh is a design note, not implemented", file=sys.stderr)
         sys.exit(2)
 
     from oplus2_matlab_amd import mesh as M

@@ -3,7 +3,8 @@
  *
  * TEST INFRASTRUCTURE ONLY.  Nothing in the product library links, loads or
  * calls this code; only the tests, the smoke check and the benchmark's CPU
- * baseline leg do, and only as the checker or as the timed CPU baseline.
+ * baseline leg do, and only as the reference results are compared against or
+ * as the timed CPU baseline.
  *
  * It restates, in plain C, the reference's sequential execution order
  * (common/op_seq.h:11-19,339-393), its execution-plan builder

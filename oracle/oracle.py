@@ -1,7 +1,7 @@
 """ctypes access to the CPU oracle (oracle/lib/liboracle.so).
 
-TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() as
-the checker, and by bench.py's cpu_baseline leg.  The product library never
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() to
+compare results against, and by bench.py's cpu_baseline leg.  The product library never
 imports it.
 
 The oracle restates the reference's sequential order (common/op_seq.h), its
