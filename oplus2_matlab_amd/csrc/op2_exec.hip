@@ -334,14 +334,34 @@ __global__ __launch_bounds__(NT) void k_res_calc_owner_direct(
   for (int j = 0; j < OWN; j++)
     if (gown[j] >= 0) Vec<T>::load4(res + 4 * (size_t)gown[j], rpre[j]);
   for (int p = tid; p < np; p += NT) {
-    const op2_v4i gv = __builtin_nontemporal_load((const op2_v4i *)(ids + p0 + p));
-    const int4 g = make_int4(gv.x, gv.y, gv.z, gv.w);
-    T x1[2], x2[2], q1[4], q2[4], f[4];
-    Vec<T>::load2(x + 2 * (size_t)g.x, x1);
-    Vec<T>::load2(x + 2 * (size_t)g.y, x2);
-    Vec<T>::load4(q + 4 * (size_t)g.z, q1);
-    Vec<T>::load4(q + 4 * (size_t)g.w, q2);
-    edge_flux(x1, x2, q1, q2, adt[g.z], adt[g.w], k, f);
+    const op2_v4i g = __builtin_nontemporal_load((const op2_v4i *)(ids + p0 + p));
+    // Each gather instruction takes the lower-numbered node (cell) of every
+    // position, the next one the higher-numbered: positions follow their
+    // smaller cell, so neighbouring lanes then read neighbouring records
+    // (fewer L1 lines per instruction than the map's slot order, whose
+    // orientation is arbitrary).  The pairs are swapped back in registers,
+    // so the flux sees the map's order.
+    const int na = min(g.x, g.y), nb = max(g.x, g.y);
+    const int ca = min(g.z, g.w), cb = max(g.z, g.w);
+    T xa[2], xb[2], qa[4], qb[4], f[4];
+    Vec<T>::load2(x + 2 * (size_t)na, xa);
+    Vec<T>::load2(x + 2 * (size_t)nb, xb);
+    Vec<T>::load4(q + 4 * (size_t)ca, qa);
+    Vec<T>::load4(q + 4 * (size_t)cb, qb);
+    const T aa = adt[ca], ab = adt[cb];
+    const bool sn = g.x != na, sc = g.z != ca;
+    T x1[2], x2[2], q1[4], q2[4];
+#pragma unroll
+    for (int d = 0; d < 2; d++) {
+      x1[d] = sn ? xb[d] : xa[d];
+      x2[d] = sn ? xa[d] : xb[d];
+    }
+#pragma unroll
+    for (int d = 0; d < 4; d++) {
+      q1[d] = sc ? qb[d] : qa[d];
+      q2[d] = sc ? qa[d] : qb[d];
+    }
+    edge_flux(x1, x2, q1, q2, sc ? ab : aa, sc ? aa : ab, k, f);
 #pragma unroll
     for (int d = 0; d < 4; d++) fs[d * np + p] = f[d];
   }
