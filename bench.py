@@ -7,6 +7,10 @@ the 2400 x 1200 O-mesh (2.88M cells, 5.76M edges), with every dat resident
 in HBM.  The iteration is captured once as a hipGraph with deferred
 reductions (the rms of every iteration lands in the runtime's device log)
 and replayed; value = cells x steps / seconds of the timed replays (FP64).
+Each graph holds --graph-iters iterations (default 8), and the capture
+fuses loops that can share a pass: update with the next iteration's
+save_soln and each update or save_soln with the adt_calc that follows
+(--no-fusion captures the loops one kernel each).
 
 roofline: res_calc, the hot loop.  achieved = algorithmic bytes of one call
 (each touched dat once per direction plus each map once: 76 B/edge FP64)
@@ -42,6 +46,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -79,6 +84,10 @@ def parse():
                     help="meshes of the batched measurement (0 = skip)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="target CPU time of the cpu_baseline sample (0 = skip)")
+    ap.add_argument("--no-fusion", dest="fusion", action="store_false",
+                    help="capture the graphs without loop fusion")
+    ap.add_argument("--graph-iters", type=int, default=8,
+                    help="iterations per captured graph (its gcd with the step count)")
     return ap.parse_args()
 
 
@@ -137,11 +146,12 @@ def measure(mesh, consts, dtype, a, renumber=None, fixed=None, steps=None):
                 n, secs, bytes_ = st
                 loops[name] = {"calls": n, "us_per_call": 1e6 * secs / n,
                                "GBs": bytes_ * n / secs / 1e9, "bytes_per_call": bytes_}
-        gid = af.capture_iteration()
+        k = math.gcd(max(a.graph_iters, 1), nsteps)
+        gid = af.capture_iteration(k)
         af.run_graph(gid, a.warmup)
         op2.sync()
         t0 = time.perf_counter()
-        op2.graph_launch(gid, nsteps)
+        op2.graph_launch(gid, nsteps // k)
         op2.sync()
         dt_graph = time.perf_counter() - t0
         log = op2.reduction_log()
@@ -152,7 +162,7 @@ def measure(mesh, consts, dtype, a, renumber=None, fixed=None, steps=None):
         op2.set_loop_config(None, 0, 0, 0)
     return {"value": mesh.ncell * nsteps / dt_graph, "ms_per_step": 1e3 * dt_graph / nsteps,
             "eager_ms_per_step": 1e3 * dt_eager / nsteps, "loops": loops, "rms_last": rms_last,
-            "tuned": tuned}
+            "tuned": tuned, "graph_iters": k}
 
 
 def measure_batch(a, batch, dtype):
@@ -180,7 +190,7 @@ def measure_batch(a, batch, dtype):
             st = op2.loop_stats(name)
             if st:
                 loops[name] = {"calls": st[0], "us_per_call": 1e6 * st[1] / st[0]}
-        gid = af.capture_iteration()
+        gid = af.capture_iteration(math.gcd(max(a.graph_iters, 1), a.steps))
         af.run_graph(gid, a.warmup)
         op2.sync()
         t0 = time.perf_counter()
@@ -253,7 +263,9 @@ def main():
         sys.exit(2)
 
     from oplus2_matlab_amd import mesh as M
+    from oplus2_matlab_amd import op2
 
+    op2.set_graph_fusion(a.fusion)
     ni, nj = (int(v) for v in a.mesh.lower().split("x"))
     mesh = M.o_mesh(ni, nj, a.ordering, seed=1)
     consts = M.flow_constants().as_array()
@@ -282,7 +294,8 @@ def main():
                                f"replayed as a hipGraph",
                    "cells": mesh.ncell, "edges": mesh.nedge, "ordering": a.ordering,
                    "strategy": a.strategy, "part_size": a.part or None,
-                   "renumbered": a.renumber == "on",
+                   "renumbered": a.renumber == "on", "graph_fusion": a.fusion,
+                   "graph_iters": r64["graph_iters"],
                    "tuned": r64["tuned"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,

@@ -291,6 +291,12 @@ int op2_graph_end(void);
 int op2_graph_launch(int id, int count);
 int op2_graph_destroy(int id);
 
+/* Loop fusion during graph capture (on by default): a captured save_soln or
+ * update followed by an adt_calc on the same cells that reads the q they
+ * write runs as one kernel (q is read once), with the same results bit for
+ * bit.  Returns the previous setting, or -1 during a capture (no change). */
+int op2_set_graph_fusion(int enable);
+
 /* Device pointer of a dat (for zero-copy interop). */
 void *op2_dat_device_ptr(op_dat dat);
 

@@ -36,6 +36,8 @@ class Airfoil:
         if res0 is not None:
             self.res[:] = np.asarray(res0, dtype=real).reshape(-1)
         self.adt = np.zeros(mesh.ncell, dtype=real)
+        self._graph_iters = {}  # graph id -> iterations it holds
+        self._single = None     # a one-iteration graph for remainders
 
         self.nodes = op_decl_set(mesh.nnode, "nodes")
         self.edges = op_decl_set(mesh.nedge, "edges")
@@ -150,39 +152,62 @@ class Airfoil:
             rms = self.update()
         return float(np.sqrt(rms / self.ncell))
 
-    def capture_iteration(self) -> int:
-        """Captures one iteration as a hipGraph (deferred reductions on; the
-        loops must have run once before).  Returns the graph id."""
+    def capture_iteration(self, iters: int = 1) -> int:
+        """Captures `iters` iterations as one hipGraph (deferred reductions
+        on; the loops must have run once before).  Returns the graph id.
+        With more than one iteration the runtime also fuses each update with
+        the next iteration's save_soln and adt_calc (graph loop fusion)."""
+        if iters < 1:
+            raise ValueError("capture at least one iteration")
         op2.set_deferred_reductions(True)
         op2.graph_begin()
         try:
-            self.save_soln()
-            for _ in range(2):
-                self.adt_calc()
-                self.res_calc()
-                self.bres_calc()
-                self.update()
+            for _ in range(iters):
+                self.save_soln()
+                for _ in range(2):
+                    self.adt_calc()
+                    self.res_calc()
+                    self.bres_calc()
+                    self.update()
         except Exception:
             op2.graph_end()
             raise
-        return op2.graph_end()
+        gid = op2.graph_end()
+        self._graph_iters[gid] = iters
+        return gid
+
+    def _replays(self, gid: int, niter: int):
+        """(graph, launches) pairs that run niter iterations: whole replays of
+        gid, then a one-iteration graph (captured on first need) for the
+        rest."""
+        k = self._graph_iters.get(gid, 1)
+        out = [(gid, niter // k)] if niter // k else []
+        if niter % k:
+            if self._single is None:
+                self._single = self.capture_iteration(1)
+            out.append((self._single, niter % k))
+        return out
 
     def run_graph(self, gid: int, niter: int, chunk: int = 8192) -> np.ndarray:
-        """Replays the captured iteration niter times; returns the rms
-        history from the reduction log (two updates per iteration; the
-        second stage's sum gives the iteration's rms)."""
+        """Runs niter iterations by replaying the captured graph; returns
+        the rms history from the reduction log (two updates per iteration;
+        the second stage's sum gives the iteration's rms)."""
         op2.reduction_log()  # drop earlier entries
         hist = []
-        done = 0
-        while done < niter:
-            n = min(chunk, niter - done)
-            op2.graph_launch(gid, n)
-            log = op2.reduction_log()
-            if log.size != 2 * n:
-                raise op2.OP2Error(f"reduction log holds {log.size} entries, expected {2 * n}")
-            hist.append(np.sqrt(log[1::2] / self.ncell))
-            done += n
-        return np.concatenate(hist)
+        for g, count in self._replays(gid, niter):
+            k = self._graph_iters[g]
+            per = max(1, chunk // k)
+            done = 0
+            while done < count:
+                n = min(per, count - done)
+                op2.graph_launch(g, n)
+                log = op2.reduction_log()
+                if log.size != 2 * n * k:
+                    raise op2.OP2Error(f"reduction log holds {log.size} entries, "
+                                       f"expected {2 * n * k}")
+                hist.append(np.sqrt(log[1::2] / self.ncell))
+                done += n
+        return np.concatenate(hist) if hist else np.zeros(0)
 
     def fetch(self):
         for d in (self.p_q, self.p_qold, self.p_adt, self.p_res):
@@ -191,17 +216,18 @@ class Airfoil:
 
 
 def run_airfoil(mesh, consts, niter: int, dtype: str = "double", graph: bool = False,
-                renumber: bool = False):
+                renumber: bool = False, graph_iters: int = 1):
     """Declares the mesh, runs `niter` iterations, returns (rms history,
     q, adt, res) after fetching the dats.  With graph=True the first
-    iteration runs eagerly and the rest replay a captured hipGraph with
-    deferred reductions.  The caller owns op_init/op_exit."""
+    iteration runs eagerly and the rest replay a captured hipGraph of
+    graph_iters iterations with deferred reductions.  The caller owns
+    op_init/op_exit."""
     a = Airfoil(mesh, consts, dtype, renumber=renumber)
     if not graph or niter < 2:
         hist = np.array([a.iterate() for _ in range(niter)])
     else:
         first = a.iterate()
-        gid = a.capture_iteration()
+        gid = a.capture_iteration(graph_iters)
         hist = np.concatenate([[first], a.run_graph(gid, niter - 1)])
         op2.set_deferred_reductions(False)
     q, _, adt, res = a.fetch()
@@ -273,18 +299,20 @@ class AirfoilBatch(Airfoil):
         return self._history(op2.reduction_log(), 1)[0]
 
     def run_graph(self, gid: int, niter: int, chunk: int = 0) -> np.ndarray:
-        """Replays the captured iteration niter times; returns the
-        (niter, nmesh) rms history."""
-        per_chunk = max(1, op2.REDUCTION_LOG_CAPACITY // (2 * self.nmesh))
-        chunk = min(chunk, per_chunk) if chunk > 0 else per_chunk
+        """Runs niter iterations by replaying the captured graph; returns
+        the (niter, nmesh) rms history."""
+        cap = max(1, op2.REDUCTION_LOG_CAPACITY // (2 * self.nmesh))
         op2.reduction_log()
         hist = []
-        done = 0
-        while done < niter:
-            n = min(chunk, niter - done)
-            op2.graph_launch(gid, n)
-            hist.append(self._history(op2.reduction_log(), n))
-            done += n
+        for g, count in self._replays(gid, niter):
+            k = self._graph_iters[g]
+            per = max(1, (min(chunk, cap) if chunk > 0 else cap) // k)
+            done = 0
+            while done < count:
+                n = min(per, count - done)
+                op2.graph_launch(g, n)
+                hist.append(self._history(op2.reduction_log(), n * k))
+                done += n
         return np.concatenate(hist) if hist else np.zeros((0, self.nmesh))
 
     def mesh_slice(self, arr: np.ndarray, i: int) -> np.ndarray:
@@ -294,7 +322,7 @@ class AirfoilBatch(Airfoil):
 
 
 def run_airfoil_batch(batch, niter: int, dtype: str = "double", graph: bool = False,
-                      renumber: bool = False):
+                      renumber: bool = False, graph_iters: int = 1):
     """Runs `niter` iterations of every mesh of the batch; returns the
     (niter, nmesh) rms history and the union's q after fetching.  The caller
     owns op_init/op_exit."""
@@ -303,7 +331,7 @@ def run_airfoil_batch(batch, niter: int, dtype: str = "double", graph: bool = Fa
         hist = np.stack([a.iterate() for _ in range(niter)]) if niter else np.zeros((0, a.nmesh))
     else:
         first = a.iterate()
-        gid = a.capture_iteration()
+        gid = a.capture_iteration(graph_iters)
         hist = np.concatenate([first[None, :], a.run_graph(gid, niter - 1)])
     op2.set_deferred_reductions(False)
     q, _, _, _ = a.fetch()

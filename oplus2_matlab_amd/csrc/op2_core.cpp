@@ -106,7 +106,11 @@ void *reduct_scratch(size_t bytes) {
     if (r.capturing) fail("op2: reduction scratch grown during graph capture");
     if (r.d_reduct) {
       OP2_HIP_CHECK(hipStreamSynchronize(r.stream));
-      dev_free(r.d_reduct);
+      // a captured graph may still use the old buffer: keep it until op_exit
+      if (r.graphs.empty())
+        dev_free(r.d_reduct);
+      else
+        r.retired.push_back(r.d_reduct);
     }
     r.reduct_bytes = ((bytes + 255) / 256) * 256 * 2;
     r.d_reduct = dev_alloc(r.reduct_bytes);
@@ -207,6 +211,7 @@ void timing_drain() {
 
 static void clear_all() {
   Runtime &r = rt();
+  fusion_drop();
   if (r.capturing) {
     hipGraph_t g = nullptr;
     (void)hipStreamEndCapture(r.stream, &g);
@@ -217,6 +222,8 @@ static void clear_all() {
   for (auto &g : r.graphs)
     if (g) (void)hipGraphExecDestroy(g);
   r.graphs.clear();
+  for (void *p : r.retired) dev_free(p);
+  r.retired.clear();
   r.deferred = false;
   for (auto &p : r.plans) p->release();
   r.plans.clear();
@@ -702,6 +709,14 @@ int op2_graph_end(void) {
   Runtime &r = rt();
   if (!r.capturing) fail("op2: no graph capture active");
   hipGraph_t g = nullptr;
+  try {
+    fusion_flush();
+  } catch (const Error &) {
+    r.capturing = false;
+    (void)hipStreamEndCapture(r.stream, &g);
+    if (g) (void)hipGraphDestroy(g);
+    throw;
+  }
   r.capturing = false;
   OP2_HIP_CHECK(hipStreamEndCapture(r.stream, &g));
   hipGraphExec_t ex = nullptr;
@@ -710,6 +725,14 @@ int op2_graph_end(void) {
   r.graphs.push_back(ex);
   return (int)r.graphs.size() - 1;
   API_END(-1)
+}
+
+int op2_set_graph_fusion(int enable) {
+  Runtime &r = rt();
+  const int prev = r.graph_fusion;
+  if (r.capturing) return -1;
+  r.graph_fusion = enable != 0;
+  return prev;
 }
 
 int op2_graph_launch(int id, int count) {

@@ -173,9 +173,12 @@ struct Runtime {
   double *d_log = nullptr;
   unsigned *d_log_count = nullptr;
   unsigned long long log_read = 0;
-  // graph capture of loop sequences on the runtime stream
+  // graph capture of loop sequences on the runtime stream; with
+  // graph_fusion, adjacent loops that can share a pass are fused there
   bool capturing = false;
+  bool graph_fusion = true;
   std::vector<hipGraphExec_t> graphs;
+  std::vector<void *> retired;  // replaced scratch buffers that graphs may use
   std::map<std::string, OnlineTune> online;
   // renumbering (op2_renumber): new number of every element of a renumbered
   // set, the starts of its independently ordered pieces, and the
@@ -196,6 +199,9 @@ void ensure_device();
 void *dev_alloc(size_t bytes);
 void dev_free(void *p);
 void release_exec_caches();
+// loop fusion during graph capture: launch, or forget, a held loop
+void fusion_flush();
+void fusion_drop();
 void h2d(void *dst, const void *src, size_t bytes);
 void d2h(void *dst, const void *src, size_t bytes);
 void ensure_map_device(op_map m);

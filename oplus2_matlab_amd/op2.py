@@ -133,6 +133,7 @@ SIGNATURES = {
     "op2_graph_end": (C.c_int, []),
     "op2_graph_launch": (C.c_int, [C.c_int, C.c_int]),
     "op2_graph_destroy": (C.c_int, [C.c_int]),
+    "op2_set_graph_fusion": (C.c_int, [C.c_int]),
 }
 
 _lib = None
@@ -444,6 +445,16 @@ def graph_launch(gid: int, count: int = 1) -> None:
 
 def graph_destroy(gid: int) -> None:
     _call("op2_graph_destroy", gid)
+
+
+def set_graph_fusion(on: bool) -> bool:
+    """Loop fusion during graph capture (default on): save_soln or update
+    followed by an adt_calc on the q they write run as one kernel.  Returns
+    the previous setting."""
+    prev = _call("op2_set_graph_fusion", int(on))
+    if prev < 0:
+        raise OP2Error("op2_set_graph_fusion: not during graph capture")
+    return bool(prev)
 
 
 def plan_summary(name: str, set_: Set, args, strategy: int = STRATEGY_COLOURED,

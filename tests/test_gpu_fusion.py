@@ -1,0 +1,135 @@
+"""Loop fusion inside graph capture: save_soln + adt_calc, update +
+adt_calc, and update + the next iteration's save_soln + adt_calc run as one
+kernel each when captured back to back.  The fused
+graph must give the unfused graph's results bit for bit (same per-cell
+arithmetic, same reduction order), and a held loop that finds no partner
+must still run, in call order."""
+
+import numpy as np
+import pytest
+
+from oplus2_matlab_amd import airfoil as AF
+from oplus2_matlab_amd import mesh as M
+from oplus2_matlab_amd import op2
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def note(msg):
+    print(f"[gpu] {msg}", flush=True)
+
+
+@pytest.fixture
+def runtime():
+    """Owner-compute strategies throughout: their sums have a fixed order,
+    so two runs can be compared bit for bit (atomic increments cannot)."""
+    op2.op_init([], 0)
+    for loop in ("res_calc", "bres_calc"):
+        op2.set_loop_config(loop, op2.STRATEGY_OWNER_GATHER, 128, 0)
+    yield
+    op2.set_graph_fusion(True)
+    op2.set_deferred_reductions(False)
+    op2.op_exit()
+    op2.set_loop_config(None, 0, 0, 0)
+
+
+def graph_run(mesh, c, dtype, niter, fusion, renumber=False, graph_iters=1):
+    op2.set_graph_fusion(fusion)
+    hist, q, adt, res = AF.run_airfoil(mesh, c, niter, dtype, graph=True, renumber=renumber,
+                                       graph_iters=graph_iters)
+    return hist, q, adt, res
+
+
+@pytest.mark.parametrize("dtype", ["double", "float"])
+@pytest.mark.parametrize("renumber,graph_iters", [(False, 1), (True, 1), (False, 3), (True, 4)])
+def test_fused_graph_equals_unfused(runtime, dtype, renumber, graph_iters):
+    """graph_iters > 1 adds the update + next save_soln + adt_calc fusion;
+    29 replayed iterations leave a remainder for the one-iteration graph."""
+    note(f"fused vs unfused graph, 40x20 random, {dtype}, renumber={renumber}, "
+         f"{graph_iters} iterations per graph, 30 iterations")
+    mesh = M.o_mesh(40, 20, "random", seed=3)
+    c = M.flow_constants().as_array()
+    h1, q1, a1, r1 = graph_run(mesh, c, dtype, 30, True, renumber, graph_iters)
+    op2.op_exit()
+    op2.op_init([], 0)
+    h0, q0, a0, r0 = graph_run(mesh, c, dtype, 30, False, renumber, graph_iters)
+    assert np.array_equal(h1, h0)
+    assert np.array_equal(q1, q0) and np.array_equal(a1, a0) and np.array_equal(r1, r0)
+    if dtype == "double":
+        st, oh = O.airfoil_seq(mesh, c, 30, "f64")
+        assert np.max(np.abs(h1 - oh) / oh) < 1e-12
+
+
+@pytest.mark.parametrize("graph_iters", [1, 4])
+def test_fused_batch_graph_equals_unfused(runtime, graph_iters):
+    note(f"fused vs unfused batch graph, 6 meshes, 12 iterations, {graph_iters} per graph")
+    batch = M.mesh_batch(nmesh=6, seed=4, cells_lo=300, cells_hi=3000, ordering="random")
+    op2.set_graph_fusion(True)
+    h1, q1 = AF.run_airfoil_batch(batch, 12, "double", graph=True, graph_iters=graph_iters)
+    op2.op_exit()
+    op2.op_init([], 0)
+    op2.set_graph_fusion(False)
+    h0, q0 = AF.run_airfoil_batch(batch, 12, "double", graph=True, graph_iters=graph_iters)
+    assert np.array_equal(h1, h0) and np.array_equal(q1, q0)
+    for i in range(batch.nmesh):
+        _, h = O.airfoil_seq(batch.meshes[i], batch.consts[i], 12, "f64")
+        assert np.max(np.abs(h1[:, i] - h) / h) < 1e-12
+
+
+def test_held_loop_without_partner_runs_in_order(runtime):
+    """save_soln held, then an adt_calc that reads qold (not the q that
+    save_soln copies): no fusion, and the copy must land before adt_calc
+    reads it.  A graph that ends on a held loop still runs it."""
+    note("held save_soln flushed before a non-matching adt_calc; graph ending on a held loop")
+    mesh = M.o_mesh(24, 12, "random", seed=6)
+    c = M.flow_constants().as_array()
+    rng = np.random.Generator(np.random.PCG64(5))
+    q0 = np.tile(c[6:10], (mesh.ncell, 1)) * (1 + 0.01 * rng.standard_normal((mesh.ncell, 4)))
+    a = AF.Airfoil(mesh, c, "double", q0=q0)
+    zeros = np.zeros(4 * mesh.ncell)
+    p_zero = op2.op_decl_dat(a.cells, 4, "double", zeros, "p_zero")
+    R, W, ID = op2.OP_READ, op2.OP_WRITE, op2.OP_ID
+
+    def save(src):
+        op2.op_par_loop(op2.save_soln, "save_soln", a.cells,
+                        op2.op_arg_dat(src, -1, ID, 4, "double", R),
+                        op2.op_arg_dat(a.p_qold, -1, ID, 4, "double", W))
+
+    def adt_on_qold():
+        op2.op_par_loop(op2.adt_calc, "adt_calc", a.cells,
+                        *[op2.op_arg_dat(a.p_x, i, a.pcell, 2, "double", R) for i in range(4)],
+                        op2.op_arg_dat(a.p_qold, -1, ID, 4, "double", R),
+                        op2.op_arg_dat(a.p_adt, -1, ID, 1, "double", W))
+
+    save(a.p_q)
+    adt_on_qold()
+    op2.set_deferred_reductions(True)
+    op2.graph_begin()
+    save(a.p_q)
+    adt_on_qold()
+    g1 = op2.graph_end()
+    op2.graph_begin()
+    save(a.p_q)
+    g2 = op2.graph_end()
+    oadt = np.zeros(mesh.ncell)
+    O.adt_calc("f64", mesh, mesh.x.copy(), q0.copy(), oadt, c)
+    for g in (g1, g2):
+        save(p_zero)  # qold = 0, so a copy that is skipped or late shows
+        op2.graph_launch(g, 1)
+        op2.sync()
+        _, qold, adt, _ = a.fetch()
+        assert np.array_equal(qold, q0), g
+        if g == g1:
+            assert np.max(np.abs(adt - oadt) / oadt) < 1e-15
+
+
+def test_fusion_switch_refused_during_capture(runtime):
+    mesh = M.o_mesh(12, 6, "random", seed=2)
+    a = AF.Airfoil(mesh, M.flow_constants().as_array(), "double")
+    a.iterate()
+    op2.set_deferred_reductions(True)
+    op2.graph_begin()
+    with pytest.raises(op2.OP2Error, match="graph capture"):
+        op2.set_graph_fusion(False)
+    op2.graph_end()
