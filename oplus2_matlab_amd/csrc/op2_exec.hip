@@ -637,29 +637,69 @@ __global__ __launch_bounds__(NT) void k_bres_calc(
 // stored to qold_out as well; ADT - an adt_calc on that q follows: the
 // cell's adt is computed from the q just formed.  qold_out may be qold and
 // adt_out may be adt (each cell reads its old values before writing).
-template <typename T, bool ADT>
-struct UpdIn {
-  T qo[4], r[4], a, xs[4][2];
+// BRES - the bres_calc that precedes update (after res_calc) is applied by
+// it: a boundary cell first adds the boundary flux of each of its boundary
+// edges to res, in edge order, exactly as bres_calc's increments would, then
+// updates.  A cell finds its boundary edges through one 8-byte record per
+// 32 cells (a bit mask of the cells with boundary edges and the number of
+// such cells in earlier records), then two 16-byte records per boundary
+// cell (nodes, edge and mesh of up to two boundary edges, copied from the
+// maps; edge -1 when absent): two dependent loads before the gathers.
+template <typename T>
+struct BresIdx {
+  const uint2 *word;  // per 32 cells: (mask, boundary cells before)
+  const int4 *rec;    // per boundary cell: 2 x (node 1, node 2, edge, mesh)
+  const T *x, *q;
+  const int *bound;
+  const T *qinf_dat;  // batched form: the free stream of each mesh
 };
 
 template <typename T, bool ADT>
+struct UpdIn {
+  T qo[4], r[4], a, xs[4][2];
+  int bidx;
+};
+
+template <typename T, bool ADT, bool BRES>
 __device__ __forceinline__ void upd_load(int c, const T *qold, const T *res, const T *adt,
-                                         const int4 *pcell, const T *x, UpdIn<T, ADT> &in) {
+                                         const int4 *pcell, const T *x, const BresIdx<T> &bi,
+                                         UpdIn<T, ADT> &in) {
   if (ADT) cell_nodes(pcell, x, c, in.xs);  // the longest chain: issued first
+  if (BRES) {
+    const uint2 w = bi.word[c >> 5];
+    const unsigned bit = 1u << (c & 31);
+    in.bidx = (w.x & bit) ? (int)(w.y + __popc(w.x & (bit - 1u))) : -1;
+  }
   Vec<T>::load4(qold + 4 * (size_t)c, in.qo);
   Vec<T>::load4(res + 4 * (size_t)c, in.r);
   in.a = adt[c];
 }
 
-template <typename T, bool ADT, bool SAVE>
+template <typename T, bool ADT, bool SAVE, bool BRES>
 __device__ __forceinline__ void upd_store(int c, const UpdIn<T, ADT> &in, T *q, T *res,
-                                          T *adt_out, T *qold_out, const AirfoilConsts<T> &k,
-                                          double &acc) {
-  T qn[4], r[4];
+                                          T *adt_out, T *qold_out, const BresIdx<T> &bi,
+                                          const AirfoilConsts<T> &k, double &acc) {
+  T qn[4], r[4] = {in.r[0], in.r[1], in.r[2], in.r[3]};
+  if (BRES && in.bidx >= 0) {
+    T qc[4];
+    Vec<T>::load4(bi.q + 4 * (size_t)c, qc);
+    const int4 rec[2] = {bi.rec[2 * in.bidx], bi.rec[2 * in.bidx + 1]};
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+      if (rec[j].z < 0) break;
+      T x1[2], x2[2], f[4];
+      Vec<T>::load2(bi.x + 2 * (size_t)rec[j].x, x1);
+      Vec<T>::load2(bi.x + 2 * (size_t)rec[j].y, x2);
+      const T *qi = bi.qinf_dat ? bi.qinf_dat + 4 * (size_t)rec[j].w : k.qinf;
+      boundary_flux(x1, x2, qc, in.a, bi.bound[rec[j].z], qi, k, f);
+#pragma unroll
+      for (int d = 0; d < 4; d++) r[d] += f[d];
+    }
+  }
   const double adti = 1.0 / (double)in.a;
 #pragma unroll
   for (int d = 0; d < 4; d++) {
-    const double del = adti * (double)in.r[d];
+    const double del = adti * (double)r[d];
     qn[d] = (T)((double)in.qo[d] - del);
     acc += del * del;
     r[d] = T(0);
@@ -677,33 +717,33 @@ __device__ __forceinline__ void upd_store(int c, const UpdIn<T, ADT> &in, T *q, 
 // cell's new state in the L2 that wrote it (measured: 180k cells, +27%
 // cell-updates/s).  Larger sets keep the plain order, which streams faster
 // (measured: 2.88M cells, 4% faster than contiguous ranges per XCD).
-template <typename T, bool ADT, bool SAVE, bool XCD>
-__global__ __launch_bounds__(NT) void k_update(const T *qold, T *__restrict__ q,
-                                               T *__restrict__ res, const T *adt, int ncell,
+template <typename T, bool ADT, bool SAVE, bool XCD, bool BRES>
+__global__ __launch_bounds__(NT) void k_update(const T *qold, T *q, T *__restrict__ res,
+                                               const T *adt, int ncell,
                                                double *__restrict__ partials,
                                                const int4 *__restrict__ pcell,
                                                const T *__restrict__ x, T *adt_out,
-                                               T *qold_out, AirfoilConsts<T> k) {
+                                               T *qold_out, BresIdx<T> bi, AirfoilConsts<T> k) {
   __shared__ double lds[NT / kWave];
   double acc = 0.0;
   const int b0 = XCD ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
   for (int c = b0 * NT + threadIdx.x; c < ncell; c += gridDim.x * NT) {
     UpdIn<T, ADT> in;
-    upd_load<T, ADT>(c, qold, res, adt, pcell, x, in);
-    upd_store<T, ADT, SAVE>(c, in, q, res, adt_out, qold_out, k, acc);
+    upd_load<T, ADT, BRES>(c, qold, res, adt, pcell, x, bi, in);
+    upd_store<T, ADT, SAVE, BRES>(c, in, q, res, adt_out, qold_out, bi, k, acc);
   }
   acc = block_reduce<kRedSum, NT>(acc, lds);
   if (threadIdx.x == 0) partials[blockIdx.x] = acc;
 }
 
-// Batched update: as k_update (ADT and SAVE included), with the squared moves summed per mesh
+// Batched update: as k_update (ADT, SAVE and BRES included), with the squared moves summed per mesh
 // (rms[cmap[c]] += ...).  cmap is non-decreasing, so each mesh is a run of
 // cells; workgroup b covers cells [b*CH, (b+1)*CH) and, for every mesh run
 // its range meets, writes one partial (block tree over its threads, in a
 // fixed order) at slot pslot[m] + b - (first workgroup of mesh m).
 // k_fold_seg adds a mesh's partials in workgroup order.
-template <typename T, int UPC, bool ADT, bool SAVE>
-__global__ __launch_bounds__(NT) void k_update_seg(const T *qold, T *__restrict__ q,
+template <typename T, int UPC, bool ADT, bool SAVE, bool BRES>
+__global__ __launch_bounds__(NT) void k_update_seg(const T *qold, T *q,
                                                    T *__restrict__ res, const T *adt,
                                                    int ncell, const int *__restrict__ cmap,
                                                    const int *__restrict__ seg_off,
@@ -713,7 +753,8 @@ __global__ __launch_bounds__(NT) void k_update_seg(const T *qold, T *__restrict_
                                                    unsigned *__restrict__ log_base,
                                                    const int4 *__restrict__ pcell,
                                                    const T *__restrict__ x, T *adt_out,
-                                                   T *qold_out, AirfoilConsts<T> k) {
+                                                   T *qold_out, BresIdx<T> bi,
+                                                   AirfoilConsts<T> k) {
   __shared__ double lds[NT / kWave];
   // reserve the log slots of this call's nseg results for k_fold_seg
   if (log_count && blockIdx.x == 0 && threadIdx.x == 0) *log_base = atomicAdd(log_count, (unsigned)nseg);
@@ -727,22 +768,9 @@ __global__ __launch_bounds__(NT) void k_update_seg(const T *qold, T *__restrict_
     const int c = c_lo + threadIdx.x + j * NT;
     s[j] = 0.0;
     if (c >= c_hi) continue;
-    T qo[4], r[4], qn[4], xs[4][2];
-    if (ADT) cell_nodes(pcell, x, c, xs);  // independent of the update: issued first
-    Vec<T>::load4(qold + 4 * (size_t)c, qo);
-    Vec<T>::load4(res + 4 * (size_t)c, r);
-    const double adti = 1.0 / (double)adt[c];
-#pragma unroll
-    for (int d = 0; d < 4; d++) {
-      const double del = adti * (double)r[d];
-      qn[d] = (T)((double)qo[d] - del);
-      s[j] += del * del;
-      r[d] = T(0);
-    }
-    Vec<T>::store4(q + 4 * (size_t)c, qn);
-    Vec<T>::store4(res + 4 * (size_t)c, r);
-    if (SAVE) Vec<T>::store4(qold_out + 4 * (size_t)c, qn);
-    if (ADT) adt_out[c] = adt_of(xs[0], xs[1], xs[2], xs[3], qn, k);
+    UpdIn<T, ADT> in;
+    upd_load<T, ADT, BRES>(c, qold, res, adt, pcell, x, bi, in);
+    upd_store<T, ADT, SAVE, BRES>(c, in, q, res, adt_out, qold_out, bi, k, s[j]);
   }
   const int m_lo = cmap[c_lo], m_hi = cmap[c_hi - 1];
   for (int m = m_lo; m <= m_hi; m++) {
@@ -1221,8 +1249,87 @@ int narrow_strategy(int s) {
   return s;
 }
 
+// Boundary edges per cell of a bres_calc (its cell map pbecell), for the
+// update that applies it: per 32 cells a (mask, boundary cells before)
+// record, and per boundary cell two (node 1, node 2, edge, mesh) records
+// from pbedge and, in the batched form, bedge -> mesh (edge -1: absent).
+// Built when bres_calc runs outside a graph capture (the capture cannot
+// copy to the device), cached per loop's maps.  `usable` is false when a
+// cell has more than two boundary edges: that bres_calc is not fused.
+struct BresIndex {
+  op_map pbedge = nullptr, pbecell = nullptr, bmesh = nullptr;
+  bool usable = false;
+  uint2 *d_word = nullptr;
+  int4 *d_rec = nullptr;
+};
+
+std::vector<BresIndex> &bres_cache() {
+  static std::vector<BresIndex> cache;
+  return cache;
+}
+
+const BresIndex *find_bres_index(int nargs, const op_arg *a) {
+  const op_map bm = nargs == 7 ? a[6].map : nullptr;
+  for (auto &b : bres_cache())
+    if (b.pbedge == a[0].map && b.pbecell == a[2].map && b.bmesh == bm) return &b;
+  return nullptr;
+}
+
+void build_bres_index(int nargs, const op_arg *a) {
+  if (find_bres_index(nargs, a) || rt().capturing) return;
+  BresIndex b;
+  b.pbedge = a[0].map;
+  b.pbecell = a[2].map;
+  b.bmesh = nargs == 7 ? a[6].map : nullptr;
+  const int ne = b.pbecell->from->size, nc = b.pbecell->to->size;
+  const int *cell = map_host(b.pbecell), *nodes = map_host(b.pbedge);
+  const int *mesh = b.bmesh ? map_host(b.bmesh) : nullptr;
+  std::vector<int> cnt(nc, 0);
+  for (int e = 0; e < ne; e++) cnt[cell[e]]++;
+  b.usable = std::all_of(cnt.begin(), cnt.end(), [](int n) { return n <= 2; });
+  if (b.usable) {
+    std::vector<uint2> word(std::max((nc + 31) / 32, 1), make_uint2(0u, 0u));
+    std::vector<int> slot(nc, -1);
+    int nbc = 0;
+    for (int c = 0; c < nc; c++) {
+      if (c % 32 == 0) word[c / 32].y = (unsigned)nbc;
+      if (!cnt[c]) continue;
+      word[c / 32].x |= 1u << (c % 32);
+      slot[c] = nbc++;
+    }
+    std::vector<int4> rec(2 * (size_t)std::max(nbc, 1), make_int4(0, 0, -1, 0));
+    std::vector<int> used(nbc, 0);
+    for (int e = 0; e < ne; e++) {  // edge order within each cell
+      const int i = slot[cell[e]];
+      rec[2 * (size_t)i + used[i]++] =
+          make_int4(nodes[2 * (size_t)e], nodes[2 * (size_t)e + 1], e, mesh ? mesh[e] : 0);
+    }
+    b.d_word = (uint2 *)dev_alloc(sizeof(uint2) * word.size());
+    h2d(b.d_word, word.data(), sizeof(uint2) * word.size());
+    b.d_rec = (int4 *)dev_alloc(sizeof(int4) * rec.size());
+    h2d(b.d_rec, rec.data(), sizeof(int4) * rec.size());
+  }
+  bres_cache().push_back(b);
+}
+
+template <typename T>
+BresIdx<T> bres_idx(const op_arg *b, int nb) {
+  BresIdx<T> bi{};
+  if (!b) return bi;
+  const BresIndex *ix = find_bres_index(nb, b);
+  if (!ix || !ix->usable) fail("op2: no boundary index for map %s", b[2].map->name);
+  bi.word = ix->d_word;
+  bi.rec = ix->d_rec;
+  bi.x = (const T *)b[0].dat->data_d;
+  bi.q = (const T *)b[2].dat->data_d;
+  bi.bound = (const int *)b[5].dat->data_d;
+  bi.qinf_dat = nb == 7 ? (const T *)b[6].dat->data_d : nullptr;
+  return bi;
+}
+
 template <typename T>
 void run_bres_calc(const char *name, op_set set, int nargs, op_arg *a) {
+  if (rt().graph_fusion) build_bres_index(nargs, a);  // for captures that fuse it
   const int requested = strategy_of(name, sizeof(T));
   const int strategy = narrow_strategy(requested);
   // batched form: the free stream of each boundary edge's mesh, through a map
@@ -1288,29 +1395,40 @@ ReduceScratch reduce_scratch(int nslots, int nparts = kReduceBlocks) {
   return {(double *)(base + 256), (double *)base};
 }
 
-template <typename T, bool XCD>
-auto update_kernel(bool adt, bool save) {
-  return adt ? (save ? k_update<T, true, true, XCD> : k_update<T, true, false, XCD>)
-             : (save ? k_update<T, false, true, XCD> : k_update<T, false, false, XCD>);
+#define OP2_UPD(A, S, X, B) k_update<T, A, S, X, B>
+template <typename T>
+auto update_kernel(bool adt, bool save, bool xcd, bool bres) {
+  decltype(&OP2_UPD(false, false, false, false)) table[16] = {
+      OP2_UPD(false, false, false, false), OP2_UPD(false, false, false, true),
+      OP2_UPD(false, false, true, false),  OP2_UPD(false, false, true, true),
+      OP2_UPD(false, true, false, false),  OP2_UPD(false, true, false, true),
+      OP2_UPD(false, true, true, false),   OP2_UPD(false, true, true, true),
+      OP2_UPD(true, false, false, false),  OP2_UPD(true, false, false, true),
+      OP2_UPD(true, false, true, false),   OP2_UPD(true, false, true, true),
+      OP2_UPD(true, true, false, false),   OP2_UPD(true, true, false, true),
+      OP2_UPD(true, true, true, false),    OP2_UPD(true, true, true, true)};
+  return table[adt * 8 + save * 4 + xcd * 2 + bres];
 }
+#undef OP2_UPD
 
-// update; fused with the save_soln of save_a and/or the adt_calc of adt_a
-// (both on the same cells and the q update writes) when given
+// update; fused with the bres_calc of bres_a (nb arguments) before it and
+// the save_soln of save_a and/or the adt_calc of adt_a after it (all on the
+// same cells and state) when given
 template <typename T>
 const double *run_update(op_set set, op_arg *a, const op_arg *adt_a = nullptr,
-                         const op_arg *save_a = nullptr) {
+                         const op_arg *save_a = nullptr, const op_arg *bres_a = nullptr,
+                         int nb = 0) {
   const int grid = grid_for(set->size, kReduceBlocks);
   ReduceScratch s = reduce_scratch(1, grid);
-  auto kern = (long long)grid * NT >= set->size ? update_kernel<T, true>(adt_a, save_a)
-                                                : update_kernel<T, false>(adt_a, save_a);
+  auto kern = update_kernel<T>(adt_a, save_a, (long long)grid * NT >= set->size, bres_a);
   hipLaunchKernelGGL(kern, dim3(grid), dim3(NT), 0, rt().stream,
                      (const T *)a[0].dat->data_d, (T *)a[1].dat->data_d, (T *)a[2].dat->data_d,
                      (const T *)a[3].dat->data_d, set->size, s.partials,
                      adt_a ? (const int4 *)adt_a[0].map->map_d : nullptr,
                      adt_a ? (const T *)adt_a[0].dat->data_d : nullptr,
                      adt_a ? (T *)adt_a[5].dat->data_d : nullptr,
-                     save_a ? (T *)save_a[1].dat->data_d : nullptr,
-                     adt_a ? airfoil_consts<T>() : AirfoilConsts<T>{});
+                     save_a ? (T *)save_a[1].dat->data_d : nullptr, bres_idx<T>(bres_a, nb),
+                     adt_a || bres_a ? airfoil_consts<T>() : AirfoilConsts<T>{});
   hipLaunchKernelGGL(k_fold<1>, dim3(1), dim3(NT), 0, rt().stream, (const double *)s.partials,
                      grid, 0u, s.out, rt().deferred ? rt().d_log : nullptr, rt().d_log_count);
   check_launch("update");
@@ -1367,7 +1485,8 @@ SegPlan &seg_plan(op_map m) {
 
 }  // namespace
 
-// Frees the cached segment plans (their maps are about to be freed).
+// Frees the cached segment plans and boundary indexes (their maps are about
+// to be freed).
 void release_exec_caches() {
   for (auto &sp : seg_cache()) {
     dev_free(sp.d_seg_off);
@@ -1376,21 +1495,35 @@ void release_exec_caches() {
     dev_free(sp.d_log_base);
   }
   seg_cache().clear();
+  for (auto &b : bres_cache()) {
+    dev_free(b.d_word);
+    dev_free(b.d_rec);
+  }
+  bres_cache().clear();
 }
 
 namespace {
 
+#define OP2_UPDS(A, S, B) k_update_seg<T, kSegUpc, A, S, B>
+template <typename T>
+auto update_seg_kernel(bool adt, bool save, bool bres) {
+  decltype(&OP2_UPDS(false, false, false)) table[8] = {
+      OP2_UPDS(false, false, false), OP2_UPDS(false, false, true), OP2_UPDS(false, true, false),
+      OP2_UPDS(false, true, true),   OP2_UPDS(true, false, false), OP2_UPDS(true, false, true),
+      OP2_UPDS(true, true, false),   OP2_UPDS(true, true, true)};
+  return table[adt * 4 + save * 2 + bres];
+}
+#undef OP2_UPDS
+
 template <typename T>
 void run_update_batch(op_set set, op_arg *a, const op_arg *adt_a = nullptr,
-                      const op_arg *save_a = nullptr) {
+                      const op_arg *save_a = nullptr, const op_arg *bres_a = nullptr,
+                      int nb = 0) {
   SegPlan &sp = seg_plan(a[4].map);
   const int ch = NT * kSegUpc;
   const int grid = (set->size + ch - 1) / ch;
   Runtime &r = rt();
-  auto kern = adt_a ? (save_a ? k_update_seg<T, kSegUpc, true, true>
-                              : k_update_seg<T, kSegUpc, true, false>)
-                    : (save_a ? k_update_seg<T, kSegUpc, false, true>
-                              : k_update_seg<T, kSegUpc, false, false>);
+  auto kern = update_seg_kernel<T>(adt_a, save_a, bres_a);
   hipLaunchKernelGGL(kern, dim3(grid), dim3(NT), 0, r.stream,
                      (const T *)a[0].dat->data_d, (T *)a[1].dat->data_d, (T *)a[2].dat->data_d,
                      (const T *)a[3].dat->data_d, set->size, (const int *)a[4].map->map_d,
@@ -1399,8 +1532,8 @@ void run_update_batch(op_set set, op_arg *a, const op_arg *adt_a = nullptr,
                      adt_a ? (const int4 *)adt_a[0].map->map_d : nullptr,
                      adt_a ? (const T *)adt_a[0].dat->data_d : nullptr,
                      adt_a ? (T *)adt_a[5].dat->data_d : nullptr,
-                     save_a ? (T *)save_a[1].dat->data_d : nullptr,
-                     adt_a ? airfoil_consts<T>() : AirfoilConsts<T>{});
+                     save_a ? (T *)save_a[1].dat->data_d : nullptr, bres_idx<T>(bres_a, nb),
+                     adt_a || bres_a ? airfoil_consts<T>() : AirfoilConsts<T>{});
   if (sp.nseg > 0)
     hipLaunchKernelGGL(k_fold_seg, dim3(sp.nseg), dim3(NT), 0, r.stream,
                        (const double *)sp.d_partials, (const int *)sp.d_pslot,
@@ -1663,23 +1796,28 @@ void online_call(const std::string &name, OnlineTune &ot, Loop l, int rb, op_set
 namespace {
 int run_loop(const char *name, op_set set, int nargs, op_arg *args);
 
-// Loop fusion inside a graph capture.  save_soln and update are held back
-// when called during capture.  A save_soln of the q a held update writes
-// (the next iteration starting) joins it; then, if the next loop is an
-// adt_calc on the same cells that reads that q (save_soln's source,
-// update's result), one kernel does them all (k_save_adt, k_update /
-// k_update_seg with ADT and SAVE), so q is not read again; any other next
-// call first launches what is held as it is.  Each cell's arithmetic, and
-// update's cells per workgroup, are those of the separate kernels, so the
-// results are the same bit for bit.
+// Loop fusion inside a graph capture.  bres_calc, save_soln and update are
+// held back when called during capture.  An update on the cells whose res a
+// held bres_calc increments takes the bres_calc in (BRES: each boundary cell
+// adds its boundary fluxes before updating); a save_soln of the q a held
+// update writes (the next iteration starting) joins it; then, if the next
+// loop is an adt_calc on the same cells that reads that q (save_soln's
+// source, update's result), one kernel does them all (k_save_adt, k_update /
+// k_update_seg with BRES, SAVE and ADT), so q is not read again; any other
+// next call first launches what is held as it is.  Each cell's arithmetic,
+// and update's cells per workgroup, are those of the separate kernels, so
+// the results are the same bit for bit, except that a cell with more than
+// one boundary edge adds their increments one by one in edge order (a
+// bres_calc strategy that sums them first may round differently).
 struct Held {
   bool active = false;
   Loop l = Loop::kUnknown;
   int rb = 0;
-  std::string name, save_name;
+  std::string name;
   op_set set = nullptr;
   std::vector<op_arg> args;       // dats, maps and dims only are used
   std::vector<op_arg> save_args;  // a save_soln joined to a held update
+  std::vector<op_arg> bres_args;  // a bres_calc taken in by a held update
 };
 
 Held &held() {
@@ -1690,16 +1828,21 @@ Held &held() {
 template <typename T>
 void launch_held(Held &h, const op_arg *adt_a) {
   const op_arg *save_a = h.save_args.empty() ? nullptr : h.save_args.data();
+  const op_arg *bres_a = h.bres_args.empty() ? nullptr : h.bres_args.data();
+  const int nb = (int)h.bres_args.size();
   switch (h.l) {
+    case Loop::kBresCalc:
+      run_bres_calc<T>(h.name.c_str(), h.set, (int)h.args.size(), h.args.data());
+      break;
     case Loop::kSaveSoln:
       if (adt_a)
         run_save_adt<T>(h.set, h.args.data(), adt_a);
       else
         run_save_soln<T>(h.set, h.args.data());
       break;
-    case Loop::kUpdate: run_update<T>(h.set, h.args.data(), adt_a, save_a); break;
+    case Loop::kUpdate: run_update<T>(h.set, h.args.data(), adt_a, save_a, bres_a, nb); break;
     case Loop::kUpdateBatch:
-      run_update_batch<T>(h.set, h.args.data(), adt_a, save_a);
+      run_update_batch<T>(h.set, h.args.data(), adt_a, save_a, bres_a, nb);
       break;
     default: fail("op2: loop %s cannot be held for fusion", h.name.c_str());
   }
@@ -1716,6 +1859,15 @@ void release_held(const op_arg *adt_a) {
     launch_held<float>(h, adt_a);
   check_launch(h.name.c_str());
   h.save_args.clear();
+  h.bres_args.clear();
+}
+
+// An update call (l, rb, set, a) on the cells whose res a held bres_calc
+// increments.
+bool takes_held_bres(Loop l, int rb, op_set set, const op_arg *a) {
+  const Held &h = held();
+  return h.active && h.l == Loop::kBresCalc && (l == Loop::kUpdate || l == Loop::kUpdateBatch) &&
+         h.rb == rb && h.args[2].map->to == set && h.args[4].dat == a[2].dat;
 }
 
 // A save_soln call (l, rb, set, a) that copies the q a held update writes.
@@ -1739,6 +1891,7 @@ void fusion_flush() { release_held(nullptr); }
 void fusion_drop() {
   held().active = false;
   held().save_args.clear();
+  held().bres_args.clear();
 }
 
 int par_loop(const char *name, op_set set, int nargs, op_arg *args) {
@@ -1796,11 +1949,21 @@ int run_loop(const char *name, op_set set, int nargs, op_arg *args) {
   }
   if (fuse && joins_held(l, rb, set, args)) {
     held().save_args.assign(args, args + nargs);
-    held().save_name = name;
+    return 0;
+  }
+  if (fuse && takes_held_bres(l, rb, set, args)) {
+    Held &h = held();
+    h.bres_args.swap(h.args);
+    h.l = l;
+    h.name = name;
+    h.set = set;
+    h.args.assign(args, args + nargs);
     return 0;
   }
   release_held(nullptr);
-  if (fuse && (l == Loop::kSaveSoln || l == Loop::kUpdate || l == Loop::kUpdateBatch)) {
+  if (fuse && (l == Loop::kSaveSoln || l == Loop::kUpdate || l == Loop::kUpdateBatch ||
+               (l == Loop::kBresCalc && find_bres_index(nargs, args) &&
+                find_bres_index(nargs, args)->usable))) {
     // results go to the reduction log (capture needs deferred reductions)
     Held &h = held();
     h.active = true;

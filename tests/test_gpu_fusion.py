@@ -1,9 +1,11 @@
-"""Loop fusion inside graph capture: save_soln + adt_calc, update +
-adt_calc, and update + the next iteration's save_soln + adt_calc run as one
-kernel each when captured back to back.  The fused
+"""Loop fusion inside graph capture: save_soln + adt_calc, and bres_calc +
+update + the next iteration's save_soln + adt_calc (or any run of these)
+as one kernel each when captured back to back.  The fused
 graph must give the unfused graph's results bit for bit (same per-cell
 arithmetic, same reduction order), and a held loop that finds no partner
 must still run, in call order."""
+
+import dataclasses
 
 import numpy as np
 import pytest
@@ -75,6 +77,40 @@ def test_fused_batch_graph_equals_unfused(runtime, graph_iters):
     for i in range(batch.nmesh):
         _, h = O.airfoil_seq(batch.meshes[i], batch.consts[i], 12, "f64")
         assert np.max(np.abs(h1[:, i] - h) / h) < 1e-12
+
+
+@pytest.mark.parametrize("dtype", ["double", "float"])
+def test_fused_boundary_cells_with_two_boundary_edges(runtime, dtype):
+    """A one-ring O-mesh: every cell has a wall and a far-field edge, so the
+    update that takes bres_calc in adds two boundary increments per cell."""
+    note(f"fused graph on a 32x1 O-mesh (two boundary edges per cell), {dtype}")
+    mesh = M.o_mesh(32, 1, "random", seed=8)
+    assert np.all(np.bincount(mesh.becell, minlength=mesh.ncell) == 2)
+    c = M.flow_constants().as_array()
+    h, q, _, _ = graph_run(mesh, c, dtype, 40, True, False, 4)
+    if dtype == "double":
+        st, oh = O.airfoil_seq(mesh, c, 40, "f64")
+        assert np.max(np.abs(h - oh) / oh) < 1e-12
+        assert np.max(np.abs(q - st.q.reshape(-1, 4))) / np.max(np.abs(st.q)) < 1e-12
+    else:
+        _, oh = O.airfoil_seq(mesh, c, 40, "mix")
+        assert np.max(np.abs(h - oh) / oh) < 1e-4
+
+
+def test_cell_with_three_boundary_edges_is_not_fused(runtime):
+    """A boundary edge listed twice gives its cell three: that bres_calc
+    keeps its own kernel in the graph, and the run still matches the oracle
+    (the duplicate's flux is added twice there too)."""
+    note("fused graph, 24x1 O-mesh with one boundary edge duplicated")
+    m = M.o_mesh(24, 1, "random", seed=9)
+    dup = lambda a: np.ascontiguousarray(np.concatenate([a, a[:1]]))
+    mesh = dataclasses.replace(m, bedge=dup(m.bedge), becell=dup(m.becell), bound=dup(m.bound))
+    assert np.bincount(mesh.becell).max() == 3
+    c = M.flow_constants().as_array()
+    h, q, _, _ = graph_run(mesh, c, "double", 20, True, False, 4)
+    st, oh = O.airfoil_seq(mesh, c, 20, "f64")
+    assert np.max(np.abs(h - oh) / oh) < 1e-12
+    assert np.max(np.abs(q - st.q.reshape(-1, 4))) / np.max(np.abs(st.q)) < 1e-12
 
 
 def test_held_loop_without_partner_runs_in_order(runtime):
