@@ -342,14 +342,56 @@ __device__ __forceinline__ void ell_sum(const T *fs, int np, ushort4 e4, T acc[4
 typedef int op2_v4i __attribute__((ext_vector_type(4)));
 typedef unsigned short op2_v4us __attribute__((ext_vector_type(4)));
 
+// A fold of reduction partials carried by the next kernel of a captured
+// graph instead of a launch of its own.  Single form: workgroup 0 of that
+// kernel sums partials[0 .. n) exactly as k_fold<1> does, into *out and the
+// log.  Batched form (nseg > 0): workgroup m < nseg sums mesh m's partials
+// exactly as k_fold_seg does.
+struct CarriedFold {
+  const double *partials;
+  int n;  // single form: number of partials (0: none)
+  double *out, *log;
+  unsigned *log_count;
+  const int *pslot;  // batched form: partial slots per mesh
+  int nseg;          // number of meshes (0: single form)
+  double *dat;       // the per-mesh dat incremented
+  const unsigned *log_base;
+};
+
+__device__ __forceinline__ bool carries_fold(const CarriedFold &f) {
+  return f.nseg > 0 ? (int)blockIdx.x < f.nseg : (f.n > 0 && blockIdx.x == 0);
+}
+
+__device__ __forceinline__ void carried_fold(const CarriedFold &f, double *lds) {
+  double acc = 0.0;
+  if (f.nseg > 0) {
+    const int m = blockIdx.x;
+    for (int i = f.pslot[m] + threadIdx.x; i < f.pslot[m + 1]; i += NT) acc += f.partials[i];
+    acc = block_reduce<kRedSum, NT>(acc, lds);
+    if (threadIdx.x == 0) {
+      f.dat[m] += acc;
+      if (f.log) f.log[(*f.log_base + m) & (kLogCap - 1)] = acc;
+    }
+  } else {
+    for (int i = threadIdx.x; i < f.n; i += NT) acc += f.partials[i];
+    acc = block_reduce<kRedSum, NT>(acc, lds);
+    if (threadIdx.x == 0) {
+      *f.out = acc;
+      if (f.log) f.log[atomicAdd(f.log_count, 1u) & (kLogCap - 1)] = acc;
+    }
+  }
+  __syncthreads();  // lds is reused by the caller
+}
+
 template <typename T, int OWN, bool ELL, bool OWNID>
 __global__ __launch_bounds__(NT) void k_res_calc_owner_direct(
     const int *__restrict__ blk_off, int nblk, const int4 *__restrict__ ids,
     const int *__restrict__ owned_off, const int *__restrict__ owned,
     const int *__restrict__ csr_off, const uint16_t *__restrict__ csr,
     const ushort4 *__restrict__ ell, const T *__restrict__ x, const T *__restrict__ q,
-    const T *__restrict__ adt, T *__restrict__ res, AirfoilConsts<T> k) {
+    const T *__restrict__ adt, T *__restrict__ res, AirfoilConsts<T> k, CarriedFold fold) {
   extern __shared__ __align__(16) unsigned char smem[];
+  if (carries_fold(fold)) carried_fold(fold, reinterpret_cast<double *>(smem));
   const int b = xcd_block(blockIdx.x, nblk);
   const int p0 = blk_off[b], np = blk_off[b + 1] - p0;
   const int o0 = owned_off[b], no = owned_off[b + 1] - o0;
@@ -1170,9 +1212,48 @@ void allow_lds(K kern, size_t lds) {
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
 }
 
+// The fold of the last update launched in a graph capture with fusion on,
+// not yet launched: the next owner-compute res_calc kernel carries it
+// (take_fold); any other launch first launches it as k_fold<1>
+// (flush_fold), which also keeps the next reduction from overwriting the
+// partials before they are summed.
+struct PendingFold {
+  bool active = false;
+  CarriedFold f{};
+};
+
+PendingFold &pending_fold() {
+  static PendingFold p;
+  return p;
+}
+
+void flush_fold() {
+  PendingFold &p = pending_fold();
+  if (!p.active) return;
+  p.active = false;
+  if (p.f.nseg > 0)
+    hipLaunchKernelGGL(k_fold_seg, dim3(p.f.nseg), dim3(NT), 0, rt().stream, p.f.partials,
+                       p.f.pslot, p.f.dat, p.f.log, p.f.log_base);
+  else
+    hipLaunchKernelGGL(k_fold<1>, dim3(1), dim3(NT), 0, rt().stream, p.f.partials, p.f.n, 0u,
+                       p.f.out, p.f.log, p.f.log_count);
+  check_launch("update");
+}
+
+// The pending fold for a kernel of nblocks workgroups (which must cover
+// every workgroup the fold needs), else none (launched on its own first).
+CarriedFold take_fold(int nblocks) {
+  PendingFold &p = pending_fold();
+  if (p.active && nblocks < std::max(p.f.nseg, 1)) flush_fold();
+  if (!p.active) return CarriedFold{};
+  p.active = false;
+  return p.f;
+}
+
 template <typename T>
 void run_res_calc(const char *name, op_set set, int nargs, op_arg *a) {
   const int strategy = strategy_of(name, sizeof(T));
+  if (strategy != OP2_STRATEGY_OWNER_GATHER) flush_fold();  // only that kernel carries it
   if (strategy == OP2_STRATEGY_OWNER) {
     Plan *p = plan_for(name, set, nargs, a, strategy, part_of(name, 256));
     const PlanGroup &nodes = p->groups[p->slots[0].first];
@@ -1212,13 +1293,17 @@ void run_res_calc(const char *name, op_set set, int nargs, op_arg *a) {
     else
       kern = own <= 1 ? OP2_G(1, false) : own <= 2 ? OP2_G(2, false) : OP2_G(4, false);
 #undef OP2_G
-    allow_lds(kern, lds);
-    hipLaunchKernelGGL(kern, dim3(p->nblocks), dim3(NT), lds, rt().stream, p->d_blk_off,
-                       p->nblocks, (const int4 *)p->d_ids, p->d_owned_off, p->d_owned,
-                       p->d_csr_off, p->d_csr, (const ushort4 *)p->d_ell,
-                       (const T *)a[0].dat->data_d,
+    allow_lds(kern, std::max(lds, (size_t)64));
+    if (p->nblocks == 0) {
+      flush_fold();
+      return;
+    }
+    hipLaunchKernelGGL(kern, dim3(p->nblocks), dim3(NT), std::max(lds, (size_t)64),
+                       rt().stream, p->d_blk_off, p->nblocks, (const int4 *)p->d_ids,
+                       p->d_owned_off, p->d_owned, p->d_csr_off, p->d_csr,
+                       (const ushort4 *)p->d_ell, (const T *)a[0].dat->data_d,
                        (const T *)a[2].dat->data_d, (const T *)a[4].dat->data_d,
-                       (T *)a[6].dat->data_d, airfoil_consts<T>());
+                       (T *)a[6].dat->data_d, airfoil_consts<T>(), take_fold(p->nblocks));
     return;
   }
   if (strategy == OP2_STRATEGY_DIRECT) {
@@ -1429,8 +1514,19 @@ const double *run_update(op_set set, op_arg *a, const op_arg *adt_a = nullptr,
                      adt_a ? (T *)adt_a[5].dat->data_d : nullptr,
                      save_a ? (T *)save_a[1].dat->data_d : nullptr, bres_idx<T>(bres_a, nb),
                      adt_a || bres_a ? airfoil_consts<T>() : AirfoilConsts<T>{});
-  hipLaunchKernelGGL(k_fold<1>, dim3(1), dim3(NT), 0, rt().stream, (const double *)s.partials,
-                     grid, 0u, s.out, rt().deferred ? rt().d_log : nullptr, rt().d_log_count);
+  Runtime &r = rt();
+  CarriedFold f{};
+  f.partials = s.partials;
+  f.n = grid;
+  f.out = s.out;
+  f.log = r.deferred ? r.d_log : nullptr;
+  f.log_count = r.d_log_count;
+  if (r.capturing && r.graph_fusion) {
+    pending_fold() = PendingFold{true, f};  // the next res_calc carries it
+  } else {
+    hipLaunchKernelGGL(k_fold<1>, dim3(1), dim3(NT), 0, r.stream, f.partials, f.n, 0u, f.out,
+                       f.log, f.log_count);
+  }
   check_launch("update");
   return s.out;
 }
@@ -1534,11 +1630,20 @@ void run_update_batch(op_set set, op_arg *a, const op_arg *adt_a = nullptr,
                      adt_a ? (T *)adt_a[5].dat->data_d : nullptr,
                      save_a ? (T *)save_a[1].dat->data_d : nullptr, bres_idx<T>(bres_a, nb),
                      adt_a || bres_a ? airfoil_consts<T>() : AirfoilConsts<T>{});
-  if (sp.nseg > 0)
-    hipLaunchKernelGGL(k_fold_seg, dim3(sp.nseg), dim3(NT), 0, r.stream,
-                       (const double *)sp.d_partials, (const int *)sp.d_pslot,
-                       (double *)a[4].dat->data_d, r.deferred ? r.d_log : nullptr,
-                       (const unsigned *)sp.d_log_base);
+  if (sp.nseg > 0) {
+    CarriedFold f{};
+    f.partials = sp.d_partials;
+    f.pslot = sp.d_pslot;
+    f.nseg = sp.nseg;
+    f.dat = (double *)a[4].dat->data_d;
+    f.log = r.deferred ? r.d_log : nullptr;
+    f.log_base = sp.d_log_base;
+    if (r.capturing && r.graph_fusion)
+      pending_fold() = PendingFold{true, f};  // the next res_calc carries it
+    else
+      hipLaunchKernelGGL(k_fold_seg, dim3(f.nseg), dim3(NT), 0, r.stream, f.partials, f.pslot,
+                         f.dat, f.log, f.log_base);
+  }
 }
 
 template <typename T>
@@ -1853,6 +1958,7 @@ void release_held(const op_arg *adt_a) {
   Held &h = held();
   if (!h.active) return;
   h.active = false;
+  flush_fold();
   if (h.rb == 8)
     launch_held<double>(h, adt_a);
   else
@@ -1887,11 +1993,15 @@ bool fuses_with_held(Loop l, int rb, op_set set, const op_arg *a) {
 }
 }  // namespace
 
-void fusion_flush() { release_held(nullptr); }
+void fusion_flush() {
+  release_held(nullptr);
+  flush_fold();
+}
 void fusion_drop() {
   held().active = false;
   held().save_args.clear();
   held().bres_args.clear();
+  pending_fold().active = false;
 }
 
 int par_loop(const char *name, op_set set, int nargs, op_arg *args) {
@@ -1920,6 +2030,7 @@ int run_loop(const char *name, op_set set, int nargs, op_arg *args) {
   const Loop l = identify(name, set, nargs, args);
   if (l == Loop::kUnknown) {
     release_held(nullptr);
+    flush_fold();
     // a loop registered by a translation unit of the generic path
     auto it = rt().loops.find(name);
     if (it == rt().loops.end())
@@ -1975,6 +2086,7 @@ int run_loop(const char *name, op_set set, int nargs, op_arg *args) {
     return 0;
   }
 
+  if (l != Loop::kResCalc) flush_fold();  // res_calc may carry it
   LoopTiming &t = timing_for(name, loop_bytes(set, nargs, args));
   hipEvent_t start;
   timing_begin(t, &start);

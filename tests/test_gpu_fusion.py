@@ -160,6 +160,45 @@ def test_held_loop_without_partner_runs_in_order(runtime):
             assert np.max(np.abs(adt - oadt) / oadt) < 1e-15
 
 
+def test_back_to_back_updates_fold_in_order(runtime):
+    """Two updates captured with nothing between them: the first one's
+    reduction (carried over to the next kernel when fusion is on) must be
+    folded before the second overwrites the partials.  Same log and state
+    as the capture without fusion, bit for bit."""
+    note("graph of update, update, res_calc: carried folds vs separate folds")
+    mesh = M.o_mesh(40, 20, "random", seed=12)
+    c = M.flow_constants().as_array()
+    out = {}
+    for fusion in (True, False):
+        op2.op_init([], 0)
+        for loop in ("res_calc", "bres_calc"):
+            op2.set_loop_config(loop, op2.STRATEGY_OWNER_GATHER, 128, 0)
+        op2.set_graph_fusion(fusion)
+        a = AF.Airfoil(mesh, c, "double")
+        a.iterate()
+        a.save_soln()
+        a.adt_calc()
+        a.res_calc()
+        op2.set_deferred_reductions(True)
+        op2.graph_begin()
+        a.update()
+        a.update()
+        a.res_calc()
+        a.update()
+        g = op2.graph_end()
+        op2.reduction_log()
+        op2.graph_launch(g, 3)
+        log = op2.reduction_log()
+        q, _, adt, res = a.fetch()
+        out[fusion] = (log, q.copy(), res.copy())
+        op2.set_deferred_reductions(False)
+        op2.op_exit()
+    op2.op_init([], 0)
+    assert out[True][0].size == 9
+    for x, y in zip(out[True], out[False]):
+        assert np.array_equal(x, y)
+
+
 def test_fusion_switch_refused_during_capture(runtime):
     mesh = M.o_mesh(12, 6, "random", seed=2)
     a = AF.Airfoil(mesh, M.flow_constants().as_array(), "double")
