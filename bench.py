@@ -243,7 +243,7 @@ def cpu_baseline(mesh, consts, target_s):
     # one iteration sizes the sample; loop_seconds excludes plan construction
     _, _, secs = oracle.airfoil_omp(mesh, consts, 1, "f64", nthreads=nthr)
     per_iter = max(float(np.sum(secs)), 1e-3)
-    iters = int(max(2, min(200, target_s / per_iter)))
+    iters = int(max(2, min(2000, target_s / per_iter)))
     _, _, secs = oracle.airfoil_omp(mesh, consts, iters, "f64", nthreads=nthr)
     t = float(np.sum(secs))
     return {"value": mesh.ncell * iters / t, "unit": "cell-updates/s", "cores": nthr,
