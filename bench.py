@@ -25,7 +25,8 @@ run after --accuracy-iters iterations (rms history relative error and
 max |q - q64|).
 
 mesh_600x300: the 180k-cell O-mesh with seed-random numbering (renumbered
-per --renumber), 200 graph steps.
+per --renumber, res_calc per --strategy as for the main mesh), 200 graph
+steps.
 
 orderings: the same 2.88M-cell mesh with every set numbered by a seed-random
 permutation, once renumbered by the runtime (op2_renumber) and once as
@@ -322,11 +323,11 @@ def main():
     if a.small_mesh:
         # the 600 x 300 O-mesh (180k cells) with seed-random numbering
         ms = M.o_mesh(600, 300, "random", seed=1)
-        r = measure(ms, consts, "double", a, renumber=a.renumber == "on", fixed=(5, 256), steps=200)
+        r = measure(ms, consts, "double", a, renumber=a.renumber == "on", steps=200)
         result["mesh_600x300"] = {
             "value": r["value"], "unit": "cell-updates/s", "ms_per_step": r["ms_per_step"],
             "cells": ms.ncell, "edges": ms.nedge, "ordering": "random",
-            "renumbered": a.renumber == "on",
+            "renumbered": a.renumber == "on", "strategy": a.strategy, "tuned": r["tuned"],
             "loops": {k: {"us_per_call": v["us_per_call"], "GBs": v["GBs"]}
                       for k, v in r["loops"].items()}}
     if a.orderings:

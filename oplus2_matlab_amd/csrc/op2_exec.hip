@@ -111,6 +111,47 @@ __global__ __launch_bounds__(NT) void k_save_adt(const int4 *__restrict__ pcell,
   adt[c] = adt_of(xs[0], xs[1], xs[2], xs[3], qc, k);
 }
 
+// A fold of reduction partials carried by the next kernel of a captured
+// graph instead of a launch of its own.  Single form: workgroup 0 of that
+// kernel sums partials[0 .. n) exactly as k_fold<1> does, into *out and the
+// log.  Batched form (nseg > 0): workgroup m < nseg sums mesh m's partials
+// exactly as k_fold_seg does.
+struct CarriedFold {
+  const double *partials;
+  int n;  // single form: number of partials (0: none)
+  double *out, *log;
+  unsigned *log_count;
+  const int *pslot;  // batched form: partial slots per mesh
+  int nseg;          // number of meshes (0: single form)
+  double *dat;       // the per-mesh dat incremented
+  const unsigned *log_base;
+};
+
+__device__ __forceinline__ bool carries_fold(const CarriedFold &f) {
+  return f.nseg > 0 ? (int)blockIdx.x < f.nseg : (f.n > 0 && blockIdx.x == 0);
+}
+
+__device__ __forceinline__ void carried_fold(const CarriedFold &f, double *lds) {
+  double acc = 0.0;
+  if (f.nseg > 0) {
+    const int m = blockIdx.x;
+    for (int i = f.pslot[m] + threadIdx.x; i < f.pslot[m + 1]; i += NT) acc += f.partials[i];
+    acc = block_reduce<kRedSum, NT>(acc, lds);
+    if (threadIdx.x == 0) {
+      f.dat[m] += acc;
+      if (f.log) f.log[(*f.log_base + m) & (kLogCap - 1)] = acc;
+    }
+  } else {
+    for (int i = threadIdx.x; i < f.n; i += NT) acc += f.partials[i];
+    acc = block_reduce<kRedSum, NT>(acc, lds);
+    if (threadIdx.x == 0) {
+      *f.out = acc;
+      if (f.log) f.log[atomicAdd(f.log_count, 1u) & (kLogCap - 1)] = acc;
+    }
+  }
+  __syncthreads();  // lds is reused by the caller
+}
+
 // Staged edge loop (res_calc).  One workgroup per plan block:
 //  1. stage x of the block's nodes and q, adt of its cells in LDS;
 //  2. compute the flux of each edge (EPT edges per thread) into registers;
@@ -127,8 +168,9 @@ __global__ __launch_bounds__(NT) void k_res_calc(
     const ushort4 *__restrict__ loc, const int *__restrict__ col_off,
     const int *__restrict__ col_base, const int *__restrict__ blk_ncol, const T *__restrict__ x,
     const T *__restrict__ q, const T *__restrict__ adt, T *__restrict__ res,
-    AirfoilConsts<T> k) {
+    AirfoilConsts<T> k, CarriedFold fold) {
   extern __shared__ __align__(16) unsigned char smem[];
+  if (carries_fold(fold)) carried_fold(fold, reinterpret_cast<double *>(smem));
   const int b = blkmap[blk_first + xcd_block(blockIdx.x, nblk)];
   const int e0 = blk_off[b], ne = blk_off[b + 1] - e0;
   const int n0 = node_off[b], nn = node_off[b + 1] - n0;
@@ -231,8 +273,9 @@ __global__ __launch_bounds__(NT) void k_res_calc_owner(
     const int *__restrict__ csr_off, const uint16_t *__restrict__ csr,
     const ushort4 *__restrict__ ell, const T *__restrict__ x,
     const T *__restrict__ q, const T *__restrict__ adt, T *__restrict__ res,
-    AirfoilConsts<T> k) {
+    AirfoilConsts<T> k, CarriedFold fold) {
   extern __shared__ __align__(16) unsigned char smem[];
+  if (carries_fold(fold)) carried_fold(fold, reinterpret_cast<double *>(smem));
   constexpr int pad = 16 / sizeof(T);
   const int b = xcd_block(blockIdx.x, nblk);
   const int p0 = blk_off[b], np = blk_off[b + 1] - p0;
@@ -341,47 +384,6 @@ __device__ __forceinline__ void ell_sum(const T *fs, int np, ushort4 e4, T acc[4
 // the gathered x, q and adt from the caches.
 typedef int op2_v4i __attribute__((ext_vector_type(4)));
 typedef unsigned short op2_v4us __attribute__((ext_vector_type(4)));
-
-// A fold of reduction partials carried by the next kernel of a captured
-// graph instead of a launch of its own.  Single form: workgroup 0 of that
-// kernel sums partials[0 .. n) exactly as k_fold<1> does, into *out and the
-// log.  Batched form (nseg > 0): workgroup m < nseg sums mesh m's partials
-// exactly as k_fold_seg does.
-struct CarriedFold {
-  const double *partials;
-  int n;  // single form: number of partials (0: none)
-  double *out, *log;
-  unsigned *log_count;
-  const int *pslot;  // batched form: partial slots per mesh
-  int nseg;          // number of meshes (0: single form)
-  double *dat;       // the per-mesh dat incremented
-  const unsigned *log_base;
-};
-
-__device__ __forceinline__ bool carries_fold(const CarriedFold &f) {
-  return f.nseg > 0 ? (int)blockIdx.x < f.nseg : (f.n > 0 && blockIdx.x == 0);
-}
-
-__device__ __forceinline__ void carried_fold(const CarriedFold &f, double *lds) {
-  double acc = 0.0;
-  if (f.nseg > 0) {
-    const int m = blockIdx.x;
-    for (int i = f.pslot[m] + threadIdx.x; i < f.pslot[m + 1]; i += NT) acc += f.partials[i];
-    acc = block_reduce<kRedSum, NT>(acc, lds);
-    if (threadIdx.x == 0) {
-      f.dat[m] += acc;
-      if (f.log) f.log[(*f.log_base + m) & (kLogCap - 1)] = acc;
-    }
-  } else {
-    for (int i = threadIdx.x; i < f.n; i += NT) acc += f.partials[i];
-    acc = block_reduce<kRedSum, NT>(acc, lds);
-    if (threadIdx.x == 0) {
-      *f.out = acc;
-      if (f.log) f.log[atomicAdd(f.log_count, 1u) & (kLogCap - 1)] = acc;
-    }
-  }
-  __syncthreads();  // lds is reused by the caller
-}
 
 template <typename T, int OWN, bool ELL, bool OWNID>
 __global__ __launch_bounds__(NT) void k_res_calc_owner_direct(
@@ -553,7 +555,11 @@ __global__ __launch_bounds__(NT) void k_res_calc_direct(const int2 *__restrict__
                                                         const T *__restrict__ q,
                                                         const T *__restrict__ adt,
                                                         T *__restrict__ res, int nedge,
-                                                        AirfoilConsts<T> k) {
+                                                        AirfoilConsts<T> k, CarriedFold fold) {
+  if (carries_fold(fold)) {
+    __shared__ double lds[NT / kWave];
+    carried_fold(fold, lds);
+  }
   const int e = xcd_block(blockIdx.x, gridDim.x) * NT + threadIdx.x;
   if (e >= nedge) return;
   const int2 n = pedge[e];
@@ -1168,55 +1174,10 @@ void run_adt_calc(op_set set, op_arg *a) {
                      airfoil_consts<T>());
 }
 
-template <typename T, int EPT>
-void launch_res_calc(Plan *p, op_arg *a, bool coloured) {
-  const PlanGroup &nodes = p->groups[p->slots[0].first];
-  const PlanGroup &cells = p->groups[p->slots[2].first];
-  const size_t lds = sizeof(T) * (2 * ((nodes.max_count + 1) & ~1) + 5 * (size_t)cells.max_count);
-  auto kern = coloured ? k_res_calc<T, EPT, true> : k_res_calc<T, EPT, false>;
-  if (lds > 64 * 1024)
-    OP2_HIP_CHECK(hipFuncSetAttribute((const void *)kern,
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  AirfoilConsts<T> k = airfoil_consts<T>();
-  for (int c = 0; c < p->ncolors; c++) {
-    const int first = p->colblk_off[c], nb = p->colblk_off[c + 1] - first;
-    if (nb == 0) continue;
-    hipLaunchKernelGGL(kern, dim3(nb), dim3(NT), lds, rt().stream, p->d_blk_off, p->d_blkmap,
-                       first, nb, nodes.d_off, nodes.d_list, cells.d_off, cells.d_list,
-                       (const ushort4 *)p->d_loc, p->d_col_off, p->d_col_base, p->d_blk_ncol,
-                       (const T *)a[0].dat->data_d, (const T *)a[2].dat->data_d,
-                       (const T *)a[4].dat->data_d, (T *)a[6].dat->data_d, k);
-  }
-}
-
-// LDS bytes of an owner-compute block: staged groups, padded, plus one
-// increment record per element position
-template <typename T>
-size_t owner_lds(const Plan *p, int node_words, int cell_words, int inc_words) {
-  constexpr int pad = 16 / sizeof(T);
-  size_t w = 0;
-  if (node_words) w += (size_t)node_words * ((p->groups[p->slots[0].first].max_count + 1) & ~1);
-  if (cell_words) {
-    const int mc = p->groups[p->owner_group].max_count;
-    w += (size_t)cell_words * mc + ((mc + pad - 1) / pad) * pad;
-  }
-  w += (size_t)inc_words * p->max_positions;
-  return w * sizeof(T);
-}
-
-template <typename K>
-void allow_lds(K kern, size_t lds) {
-  if (lds > 160 * 1024) fail("op2: block needs %zu bytes of LDS; reduce the part size", lds);
-  if (lds > 64 * 1024)
-    OP2_HIP_CHECK(hipFuncSetAttribute((const void *)kern,
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-}
-
 // The fold of the last update launched in a graph capture with fusion on,
-// not yet launched: the next owner-compute res_calc kernel carries it
-// (take_fold); any other launch first launches it as k_fold<1>
-// (flush_fold), which also keeps the next reduction from overwriting the
-// partials before they are summed.
+// not yet launched: the next res_calc kernel carries it (take_fold); any
+// other launch first launches it on its own (flush_fold), which also keeps
+// the next reduction from overwriting the partials before they are summed.
 struct PendingFold {
   bool active = false;
   CarriedFold f{};
@@ -1250,10 +1211,57 @@ CarriedFold take_fold(int nblocks) {
   return p.f;
 }
 
+template <typename T, int EPT>
+void launch_res_calc(Plan *p, op_arg *a, bool coloured) {
+  const PlanGroup &nodes = p->groups[p->slots[0].first];
+  const PlanGroup &cells = p->groups[p->slots[2].first];
+  const size_t lds = sizeof(T) * (2 * ((nodes.max_count + 1) & ~1) + 5 * (size_t)cells.max_count);
+  auto kern = coloured ? k_res_calc<T, EPT, true> : k_res_calc<T, EPT, false>;
+  if (lds > 64 * 1024)
+    OP2_HIP_CHECK(hipFuncSetAttribute((const void *)kern,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  AirfoilConsts<T> k = airfoil_consts<T>();
+  bool first_launch = true;
+  for (int c = 0; c < p->ncolors; c++) {
+    const int first = p->colblk_off[c], nb = p->colblk_off[c + 1] - first;
+    if (nb == 0) continue;
+    const CarriedFold fold = first_launch ? take_fold(nb) : CarriedFold{};
+    first_launch = false;
+    hipLaunchKernelGGL(kern, dim3(nb), dim3(NT), lds, rt().stream, p->d_blk_off, p->d_blkmap,
+                       first, nb, nodes.d_off, nodes.d_list, cells.d_off, cells.d_list,
+                       (const ushort4 *)p->d_loc, p->d_col_off, p->d_col_base, p->d_blk_ncol,
+                       (const T *)a[0].dat->data_d, (const T *)a[2].dat->data_d,
+                       (const T *)a[4].dat->data_d, (T *)a[6].dat->data_d, k, fold);
+  }
+  flush_fold();  // a plan without blocks carries nothing
+}
+
+// LDS bytes of an owner-compute block: staged groups, padded, plus one
+// increment record per element position
+template <typename T>
+size_t owner_lds(const Plan *p, int node_words, int cell_words, int inc_words) {
+  constexpr int pad = 16 / sizeof(T);
+  size_t w = 0;
+  if (node_words) w += (size_t)node_words * ((p->groups[p->slots[0].first].max_count + 1) & ~1);
+  if (cell_words) {
+    const int mc = p->groups[p->owner_group].max_count;
+    w += (size_t)cell_words * mc + ((mc + pad - 1) / pad) * pad;
+  }
+  w += (size_t)inc_words * p->max_positions;
+  return w * sizeof(T);
+}
+
+template <typename K>
+void allow_lds(K kern, size_t lds) {
+  if (lds > 160 * 1024) fail("op2: block needs %zu bytes of LDS; reduce the part size", lds);
+  if (lds > 64 * 1024)
+    OP2_HIP_CHECK(hipFuncSetAttribute((const void *)kern,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+}
+
 template <typename T>
 void run_res_calc(const char *name, op_set set, int nargs, op_arg *a) {
   const int strategy = strategy_of(name, sizeof(T));
-  if (strategy != OP2_STRATEGY_OWNER_GATHER) flush_fold();  // only that kernel carries it
   if (strategy == OP2_STRATEGY_OWNER) {
     Plan *p = plan_for(name, set, nargs, a, strategy, part_of(name, 256));
     const PlanGroup &nodes = p->groups[p->slots[0].first];
@@ -1268,14 +1276,18 @@ void run_res_calc(const char *name, op_set set, int nargs, op_arg *a) {
                     : (own <= 1 ? k_res_calc_owner<T, 1, false>
                                 : own <= 2 ? k_res_calc_owner<T, 2, false>
                                            : k_res_calc_owner<T, 4, false>);
-    allow_lds(kern, lds);
-    hipLaunchKernelGGL(kern, dim3(p->nblocks), dim3(NT), lds, rt().stream,
-                       p->d_blk_off, p->nblocks, nodes.d_off, nodes.d_list, cells.d_off,
-                       cells.d_list, (const ushort4 *)p->d_loc, p->d_owned_off, p->d_owned,
-                       p->d_csr_off, p->d_csr, (const ushort4 *)p->d_ell,
-                       (const T *)a[0].dat->data_d,
-                       (const T *)a[2].dat->data_d, (const T *)a[4].dat->data_d,
-                       (T *)a[6].dat->data_d, airfoil_consts<T>());
+    allow_lds(kern, std::max(lds, (size_t)64));
+    if (p->nblocks == 0) {
+      flush_fold();
+      return;
+    }
+    hipLaunchKernelGGL(kern, dim3(p->nblocks), dim3(NT), std::max(lds, (size_t)64),
+                       rt().stream, p->d_blk_off, p->nblocks, nodes.d_off, nodes.d_list,
+                       cells.d_off, cells.d_list, (const ushort4 *)p->d_loc, p->d_owned_off,
+                       p->d_owned, p->d_csr_off, p->d_csr, (const ushort4 *)p->d_ell,
+                       (const T *)a[0].dat->data_d, (const T *)a[2].dat->data_d,
+                       (const T *)a[4].dat->data_d, (T *)a[6].dat->data_d, airfoil_consts<T>(),
+                       take_fold(p->nblocks));
     return;
   }
   if (strategy == OP2_STRATEGY_OWNER_GATHER) {
@@ -1307,11 +1319,12 @@ void run_res_calc(const char *name, op_set set, int nargs, op_arg *a) {
     return;
   }
   if (strategy == OP2_STRATEGY_DIRECT) {
-    hipLaunchKernelGGL(k_res_calc_direct<T>, dim3(grid_for(set->size, 1 << 30)), dim3(NT), 0,
-                       rt().stream, (const int2 *)a[0].map->map_d, (const int2 *)a[2].map->map_d,
+    const int grid = grid_for(set->size, 1 << 30);
+    hipLaunchKernelGGL(k_res_calc_direct<T>, dim3(grid), dim3(NT), 0, rt().stream,
+                       (const int2 *)a[0].map->map_d, (const int2 *)a[2].map->map_d,
                        (const T *)a[0].dat->data_d, (const T *)a[2].dat->data_d,
                        (const T *)a[4].dat->data_d, (T *)a[6].dat->data_d, set->size,
-                       airfoil_consts<T>());
+                       airfoil_consts<T>(), take_fold(grid));
     return;
   }
   const int part = part_of(name, sizeof(T) == 8 ? 256 : 512);

@@ -63,6 +63,22 @@ def test_fused_graph_equals_unfused(runtime, dtype, renumber, graph_iters):
         assert np.max(np.abs(h1 - oh) / oh) < 1e-12
 
 
+@pytest.mark.parametrize("strategy", [op2.STRATEGY_OWNER, op2.STRATEGY_COLOURED])
+def test_fused_equals_unfused_other_strategies(runtime, strategy):
+    """The carried fold rides in whichever res_calc kernel follows: the
+    staged owner kernel, or the first launch of the coloured one."""
+    note(f"fused vs unfused graph, res_calc strategy {strategy}, 2 iterations per graph")
+    for loop in ("res_calc", "bres_calc"):
+        op2.set_loop_config(loop, strategy, 128, 0)
+    mesh = M.o_mesh(40, 20, "random", seed=13)
+    c = M.flow_constants().as_array()
+    h1, q1, a1, _ = graph_run(mesh, c, "double", 21, True, False, 2)
+    op2.op_exit()
+    op2.op_init([], 0)
+    h0, q0, a0, _ = graph_run(mesh, c, "double", 21, False, False, 2)
+    assert np.array_equal(h1, h0) and np.array_equal(q1, q0) and np.array_equal(a1, a0)
+
+
 @pytest.mark.parametrize("graph_iters", [1, 4])
 def test_fused_batch_graph_equals_unfused(runtime, graph_iters):
     note(f"fused vs unfused batch graph, 6 meshes, 12 iterations, {graph_iters} per graph")
