@@ -93,3 +93,28 @@ def test_jacobi_restart_with_max_reduction(tmp_path, golden_dir):
         op2.op_exit()
         op2.set_loop_config(None, 0, 0, 0)
     assert np.array_equal(s1, s2) and np.array_equal(m1, m2) and np.array_equal(u1, u2)
+
+
+def test_capture_before_checkpoint_is_refused(tmp_path):
+    """Loops skipped by a restart launch nothing, so capturing them into a
+    graph would record nothing: the runtime refuses."""
+    note("restart: graph capture before the checkpoint is refused")
+    path = str(tmp_path / "cap.ck")
+    mesh = M.o_mesh(24, 12, "random", seed=3)
+    c = M.flow_constants().as_array()
+    op2.op_init([], 0)
+    op2.checkpoint_record(path)
+    airfoil_run(mesh, c, 4, (3,))
+    op2.op_exit()
+    op2.op_init([], 0)
+    try:
+        op2.checkpoint_restart(path)
+        a = AF.Airfoil(mesh, c, "double")
+        a.iterate()  # skipped: before the checkpoint
+        op2.set_deferred_reductions(True)
+        with pytest.raises(op2.OP2Error, match="before the checkpoint"):
+            a.capture_iteration()
+    finally:
+        op2.set_deferred_reductions(False)
+        op2.op_exit()
+        op2.set_loop_config(None, 0, 0, 0)
