@@ -758,13 +758,13 @@ __device__ __forceinline__ void upd_store(int c, const UpdIn<T, ADT> &in, T *q, 
   if (ADT) adt_out[c] = adt_of(in.xs[0], in.xs[1], in.xs[2], in.xs[3], qn, k);
 }
 
-// One pass of update (with SAVE and ADT as above).  Workgroups stride over
-// the cells; when one pass of the grid covers every cell (XCD), each XCD
-// takes a contiguous range of the cells, as the edge and cell loops do, so
-// that at sizes where the dats stay in the caches the next loops find a
-// cell's new state in the L2 that wrote it (measured: 180k cells, +27%
-// cell-updates/s).  Larger sets keep the plain order, which streams faster
-// (measured: 2.88M cells, 4% faster than contiguous ranges per XCD).
+// One pass of update (with BRES, SAVE and ADT as above).  Workgroups
+// stride over the cells.  XCD: each XCD takes a contiguous range of the
+// cells, as the edge and cell loops do, so the next loops find a cell's new
+// state in the L2 that wrote it; used when one pass of the grid covers the
+// set (measured: 180k cells, +27% cell-updates/s) and for float state
+// (2.88M cells, FP32 form +2.5%).  Large FP64 sets keep the plain
+// interleaved order, which streams faster there (2.88M cells: 1.6-4%).
 template <typename T, bool ADT, bool SAVE, bool XCD, bool BRES>
 __global__ __launch_bounds__(NT) void k_update(const T *qold, T *q, T *__restrict__ res,
                                                const T *adt, int ncell,
@@ -774,11 +774,27 @@ __global__ __launch_bounds__(NT) void k_update(const T *qold, T *q, T *__restric
                                                T *qold_out, BresIdx<T> bi, AirfoilConsts<T> k) {
   __shared__ double lds[NT / kWave];
   double acc = 0.0;
-  const int b0 = XCD ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
-  for (int c = b0 * NT + threadIdx.x; c < ncell; c += gridDim.x * NT) {
-    UpdIn<T, ADT> in;
-    upd_load<T, ADT, BRES>(c, qold, res, adt, pcell, x, bi, in);
-    upd_store<T, ADT, SAVE, BRES>(c, in, q, res, adt_out, qold_out, bi, k, acc);
+  if (XCD) {
+    // each XCD strides over its own contiguous range of cell chunks (the
+    // ranges the edge and cell loops give the XCDs); with one pass of the
+    // grid this is xcd_block's mapping
+    const int xd = blockIdx.x % kXcds, i = blockIdx.x / kXcds;
+    const int nwg = gridDim.x / kXcds + (xd < (int)gridDim.x % kXcds ? 1 : 0);
+    const int nch = (ncell + NT - 1) / NT, base = nch / kXcds, rem = nch % kXcds;
+    const int lo = xd * base + min(xd, rem), hi = lo + base + (xd < rem ? 1 : 0);
+    for (int ch = lo + i; ch < hi; ch += nwg) {
+      const int c = ch * NT + threadIdx.x;
+      if (c >= ncell) continue;
+      UpdIn<T, ADT> in;
+      upd_load<T, ADT, BRES>(c, qold, res, adt, pcell, x, bi, in);
+      upd_store<T, ADT, SAVE, BRES>(c, in, q, res, adt_out, qold_out, bi, k, acc);
+    }
+  } else {
+    for (int c = blockIdx.x * NT + threadIdx.x; c < ncell; c += gridDim.x * NT) {
+      UpdIn<T, ADT> in;
+      upd_load<T, ADT, BRES>(c, qold, res, adt, pcell, x, bi, in);
+      upd_store<T, ADT, SAVE, BRES>(c, in, q, res, adt_out, qold_out, bi, k, acc);
+    }
   }
   acc = block_reduce<kRedSum, NT>(acc, lds);
   if (threadIdx.x == 0) partials[blockIdx.x] = acc;
@@ -1518,7 +1534,8 @@ const double *run_update(op_set set, op_arg *a, const op_arg *adt_a = nullptr,
                          int nb = 0) {
   const int grid = grid_for(set->size, kReduceBlocks);
   ReduceScratch s = reduce_scratch(1, grid);
-  auto kern = update_kernel<T>(adt_a, save_a, (long long)grid * NT >= set->size, bres_a);
+  const bool xcd = sizeof(T) == 4 || (long long)grid * NT >= set->size;
+  auto kern = update_kernel<T>(adt_a, save_a, xcd, bres_a);
   hipLaunchKernelGGL(kern, dim3(grid), dim3(NT), 0, rt().stream,
                      (const T *)a[0].dat->data_d, (T *)a[1].dat->data_d, (T *)a[2].dat->data_d,
                      (const T *)a[3].dat->data_d, set->size, s.partials,
