@@ -13,6 +13,7 @@
 // precision form: float state and flux arithmetic, with update's
 // qold - res/adt and every global reduction carried in double.
 #include <algorithm>
+#include <set>
 #include <cstring>
 
 #include "op2_internal.h"
@@ -273,7 +274,7 @@ __global__ __launch_bounds__(NT) void k_res_calc_owner(
     const int *__restrict__ csr_off, const uint16_t *__restrict__ csr,
     const ushort4 *__restrict__ ell, const T *__restrict__ x,
     const T *__restrict__ q, const T *__restrict__ adt, T *__restrict__ res,
-    AirfoilConsts<T> k, CarriedFold fold) {
+    AirfoilConsts<T> k, CarriedFold fold, bool zero_res) {
   extern __shared__ __align__(16) unsigned char smem[];
   if (carries_fold(fold)) carried_fold(fold, reinterpret_cast<double *>(smem));
   constexpr int pad = 16 / sizeof(T);
@@ -297,7 +298,12 @@ __global__ __launch_bounds__(NT) void k_res_calc_owner(
     const int i = tid + j * NT;
     gown[j] = i < no ? owned[o0 + i] : -1;
     e4[j] = ELL && i < no ? ell[o0 + i] : make_ushort4(0xffff, 0xffff, 0xffff, 0xffff);
-    if (gown[j] >= 0) Vec<T>::load4(res + 4 * (size_t)gown[j], rpre[j]);
+    if (zero_res) {  // res known to hold zeros (as in k_res_calc_owner_direct)
+#pragma unroll
+      for (int d = 0; d < 4; d++) rpre[j][d] = T(0);
+    } else if (gown[j] >= 0) {
+      Vec<T>::load4(res + 4 * (size_t)gown[j], rpre[j]);
+    }
   }
   for (int i = tid; i < nn; i += NT) {
     T v[2];
@@ -391,7 +397,8 @@ __global__ __launch_bounds__(NT) void k_res_calc_owner_direct(
     const int *__restrict__ owned_off, const int *__restrict__ owned,
     const int *__restrict__ csr_off, const uint16_t *__restrict__ csr,
     const ushort4 *__restrict__ ell, const T *__restrict__ x, const T *__restrict__ q,
-    const T *__restrict__ adt, T *__restrict__ res, AirfoilConsts<T> k, CarriedFold fold) {
+    const T *__restrict__ adt, T *__restrict__ res, AirfoilConsts<T> k, CarriedFold fold,
+    bool zero_res) {
   extern __shared__ __align__(16) unsigned char smem[];
   if (carries_fold(fold)) carried_fold(fold, reinterpret_cast<double *>(smem));
   const int b = xcd_block(blockIdx.x, nblk);
@@ -414,10 +421,18 @@ __global__ __launch_bounds__(NT) void k_res_calc_owner_direct(
       }
     }
   }
+  // zero_res: res is known to hold zeros (a captured graph's update wrote
+  // them), so its read is skipped; the sum still starts from +0
   T rpre[OWN][4];
 #pragma unroll
-  for (int j = 0; j < OWN; j++)
-    if (gown[j] >= 0) Vec<T>::load4(res + 4 * (size_t)gown[j], rpre[j]);
+  for (int j = 0; j < OWN; j++) {
+    if (zero_res) {
+#pragma unroll
+      for (int d = 0; d < 4; d++) rpre[j][d] = T(0);
+    } else if (gown[j] >= 0) {
+      Vec<T>::load4(res + 4 * (size_t)gown[j], rpre[j]);
+    }
+  }
   for (int p = tid; p < np; p += NT) {
     const op2_v4i g = __builtin_nontemporal_load((const op2_v4i *)(ids + p0 + p));
     // Each gather instruction takes the lower-numbered node (cell) of every
@@ -1190,6 +1205,27 @@ void run_adt_calc(op_set set, op_arg *a) {
                      airfoil_consts<T>());
 }
 
+// Dats known to hold zeros at this point of a graph capture with fusion on
+// (an update launched earlier in the capture zeroed its res, and nothing
+// has written it since), so that the next res_calc need not read them.
+// Empty at the start of every capture: a graph may be replayed after any
+// eager work.
+std::set<op_dat> &zero_dats() {
+  static std::set<op_dat> z;
+  return z;
+}
+
+bool known_zero(op_dat d) {
+  const Runtime &r = rt();
+  return r.capturing && r.graph_fusion && zero_dats().count(d);
+}
+
+// A launched loop wrote the dats of its non-READ arguments.
+void note_writes(int nargs, const op_arg *a) {
+  for (int m = 0; m < nargs; m++)
+    if (a[m].argtype == OP_ARG_DAT && a[m].acc != OP_READ) zero_dats().erase(a[m].dat);
+}
+
 // The fold of the last update launched in a graph capture with fusion on,
 // not yet launched: the next res_calc kernel carries it (take_fold); any
 // other launch first launches it on its own (flush_fold), which also keeps
@@ -1303,7 +1339,7 @@ void run_res_calc(const char *name, op_set set, int nargs, op_arg *a) {
                        p->d_owned, p->d_csr_off, p->d_csr, (const ushort4 *)p->d_ell,
                        (const T *)a[0].dat->data_d, (const T *)a[2].dat->data_d,
                        (const T *)a[4].dat->data_d, (T *)a[6].dat->data_d, airfoil_consts<T>(),
-                       take_fold(p->nblocks));
+                       take_fold(p->nblocks), known_zero(a[6].dat));
     return;
   }
   if (strategy == OP2_STRATEGY_OWNER_GATHER) {
@@ -1331,7 +1367,8 @@ void run_res_calc(const char *name, op_set set, int nargs, op_arg *a) {
                        p->d_owned_off, p->d_owned, p->d_csr_off, p->d_csr,
                        (const ushort4 *)p->d_ell, (const T *)a[0].dat->data_d,
                        (const T *)a[2].dat->data_d, (const T *)a[4].dat->data_d,
-                       (T *)a[6].dat->data_d, airfoil_consts<T>(), take_fold(p->nblocks));
+                       (T *)a[6].dat->data_d, airfoil_consts<T>(), take_fold(p->nblocks),
+                       known_zero(a[6].dat));
     return;
   }
   if (strategy == OP2_STRATEGY_DIRECT) {
@@ -1994,6 +2031,11 @@ void release_held(const op_arg *adt_a) {
   else
     launch_held<float>(h, adt_a);
   check_launch(h.name.c_str());
+  note_writes((int)h.bres_args.size(), h.bres_args.data());
+  note_writes((int)h.args.size(), h.args.data());
+  note_writes((int)h.save_args.size(), h.save_args.data());
+  if (adt_a) note_writes(6, adt_a);
+  if (h.l == Loop::kUpdate || h.l == Loop::kUpdateBatch) zero_dats().insert(h.args[2].dat);
   h.save_args.clear();
   h.bres_args.clear();
 }
@@ -2026,12 +2068,14 @@ bool fuses_with_held(Loop l, int rb, op_set set, const op_arg *a) {
 void fusion_flush() {
   release_held(nullptr);
   flush_fold();
+  zero_dats().clear();
 }
 void fusion_drop() {
   held().active = false;
   held().save_args.clear();
   held().bres_args.clear();
   pending_fold().active = false;
+  zero_dats().clear();
 }
 
 int par_loop(const char *name, op_set set, int nargs, op_arg *args) {
@@ -2072,6 +2116,7 @@ int run_loop(const char *name, op_set set, int nargs, op_arg *args) {
     const int rc = it->second(name, set, nargs, args);
     if (rc != 0) fail("op_par_loop error -- generic loop \"%s\": %s", name, rt().last_error.c_str());
     check_launch(name);
+    note_writes(nargs, args);
     timing_end(t, start);
     return 0;
   }
@@ -2122,6 +2167,7 @@ int run_loop(const char *name, op_set set, int nargs, op_arg *args) {
   timing_begin(t, &start);
   if (l == Loop::kUpdate) {
     const double *d = rb == 8 ? run_update<double>(set, args) : run_update<float>(set, args);
+    note_writes(nargs, args);
     timing_end(t, start);
     if (rt().deferred) return 0;
     gbl_inc(args[4], fetch_results(d, 1)[0]);
@@ -2133,11 +2179,13 @@ int run_loop(const char *name, op_set set, int nargs, op_arg *args) {
     else
       run_update_batch<float>(set, args);
     check_launch(name);
+    note_writes(nargs, args);
     timing_end(t, start);
     return 0;
   }
   if (l == Loop::kJacUpdate) {
     const double *d = rb == 8 ? run_jac_update<double>(set, args) : run_jac_update<float>(set, args);
+    note_writes(nargs, args);
     timing_end(t, start);
     if (rt().deferred) return 0;
     const double *h = fetch_results(d, 2);
@@ -2155,6 +2203,7 @@ int run_loop(const char *name, op_set set, int nargs, op_arg *args) {
       dispatch<float>(l, name, set, nargs, args);
   }
   check_launch(name);
+  note_writes(nargs, args);  // after the launch: res_calc checked its res first
   timing_end(t, start);
   return 0;
 }
