@@ -715,6 +715,9 @@ struct BresIdx {
   const T *x, *q;
   const int *bound;
   const T *qinf_dat;  // batched form: the free stream of each mesh
+  // the zeros update leaves in res are not stored: the res_calc that follows
+  // in the graph overwrites every cell's res without reading it
+  bool keep_res;
 };
 
 template <typename T, bool ADT>
@@ -768,7 +771,7 @@ __device__ __forceinline__ void upd_store(int c, const UpdIn<T, ADT> &in, T *q, 
     r[d] = T(0);
   }
   Vec<T>::store4(q + 4 * (size_t)c, qn);
-  Vec<T>::store4(res + 4 * (size_t)c, r);
+  if (!bi.keep_res) Vec<T>::store4(res + 4 * (size_t)c, r);
   if (SAVE) Vec<T>::store4(qold_out + 4 * (size_t)c, qn);
   if (ADT) adt_out[c] = adt_of(in.xs[0], in.xs[1], in.xs[2], in.xs[3], qn, k);
 }
@@ -1464,8 +1467,9 @@ void build_bres_index(int nargs, const op_arg *a) {
 }
 
 template <typename T>
-BresIdx<T> bres_idx(const op_arg *b, int nb) {
+BresIdx<T> bres_idx(const op_arg *b, int nb, bool keep_res) {
   BresIdx<T> bi{};
+  bi.keep_res = keep_res;
   if (!b) return bi;
   const BresIndex *ix = find_bres_index(nb, b);
   if (!ix || !ix->usable) fail("op2: no boundary index for map %s", b[2].map->name);
@@ -1568,7 +1572,7 @@ auto update_kernel(bool adt, bool save, bool xcd, bool bres) {
 template <typename T>
 const double *run_update(op_set set, op_arg *a, const op_arg *adt_a = nullptr,
                          const op_arg *save_a = nullptr, const op_arg *bres_a = nullptr,
-                         int nb = 0) {
+                         int nb = 0, bool keep_res = false) {
   const int grid = grid_for(set->size, kReduceBlocks);
   ReduceScratch s = reduce_scratch(1, grid);
   const bool xcd = sizeof(T) == 4 || (long long)grid * NT >= set->size;
@@ -1579,7 +1583,8 @@ const double *run_update(op_set set, op_arg *a, const op_arg *adt_a = nullptr,
                      adt_a ? (const int4 *)adt_a[0].map->map_d : nullptr,
                      adt_a ? (const T *)adt_a[0].dat->data_d : nullptr,
                      adt_a ? (T *)adt_a[5].dat->data_d : nullptr,
-                     save_a ? (T *)save_a[1].dat->data_d : nullptr, bres_idx<T>(bres_a, nb),
+                     save_a ? (T *)save_a[1].dat->data_d : nullptr,
+                     bres_idx<T>(bres_a, nb, keep_res),
                      adt_a || bres_a ? airfoil_consts<T>() : AirfoilConsts<T>{});
   Runtime &r = rt();
   CarriedFold f{};
@@ -1681,7 +1686,7 @@ auto update_seg_kernel(bool adt, bool save, bool bres) {
 template <typename T>
 void run_update_batch(op_set set, op_arg *a, const op_arg *adt_a = nullptr,
                       const op_arg *save_a = nullptr, const op_arg *bres_a = nullptr,
-                      int nb = 0) {
+                      int nb = 0, bool keep_res = false) {
   SegPlan &sp = seg_plan(a[4].map);
   const int ch = NT * kSegUpc;
   const int grid = (set->size + ch - 1) / ch;
@@ -1695,7 +1700,8 @@ void run_update_batch(op_set set, op_arg *a, const op_arg *adt_a = nullptr,
                      adt_a ? (const int4 *)adt_a[0].map->map_d : nullptr,
                      adt_a ? (const T *)adt_a[0].dat->data_d : nullptr,
                      adt_a ? (T *)adt_a[5].dat->data_d : nullptr,
-                     save_a ? (T *)save_a[1].dat->data_d : nullptr, bres_idx<T>(bres_a, nb),
+                     save_a ? (T *)save_a[1].dat->data_d : nullptr,
+                     bres_idx<T>(bres_a, nb, keep_res),
                      adt_a || bres_a ? airfoil_consts<T>() : AirfoilConsts<T>{});
   if (sp.nseg > 0) {
     CarriedFold f{};
@@ -1976,7 +1982,11 @@ int run_loop(const char *name, op_set set, int nargs, op_arg *args);
 // loop is an adt_calc on the same cells that reads that q (save_soln's
 // source, update's result), one kernel does them all (k_save_adt, k_update /
 // k_update_seg with BRES, SAVE and ADT), so q is not read again; any other
-// next call first launches what is held as it is.  Each cell's arithmetic,
+// next call first launches what is held as it is.  An update completed by
+// its adt_calc stays held for one more call: when that is a res_calc whose
+// owner-compute tiles own every cell of the res update zeroes, update does
+// not store the zeros (keep_res) and res_calc takes res as zero without
+// reading it (the zeros are never observed).  Each cell's arithmetic,
 // and update's cells per workgroup, are those of the separate kernels, so
 // the results are the same bit for bit, except that a cell with more than
 // one boundary edge adds their increments one by one in edge order (a
@@ -1990,6 +2000,7 @@ struct Held {
   std::vector<op_arg> args;       // dats, maps and dims only are used
   std::vector<op_arg> save_args;  // a save_soln joined to a held update
   std::vector<op_arg> bres_args;  // a bres_calc taken in by a held update
+  std::vector<op_arg> adt_args;   // the adt_calc that completes a held update
 };
 
 Held &held() {
@@ -1998,7 +2009,7 @@ Held &held() {
 }
 
 template <typename T>
-void launch_held(Held &h, const op_arg *adt_a) {
+void launch_held(Held &h, const op_arg *adt_a, bool keep_res) {
   const op_arg *save_a = h.save_args.empty() ? nullptr : h.save_args.data();
   const op_arg *bres_a = h.bres_args.empty() ? nullptr : h.bres_args.data();
   const int nb = (int)h.bres_args.size();
@@ -2012,24 +2023,31 @@ void launch_held(Held &h, const op_arg *adt_a) {
       else
         run_save_soln<T>(h.set, h.args.data());
       break;
-    case Loop::kUpdate: run_update<T>(h.set, h.args.data(), adt_a, save_a, bres_a, nb); break;
+    case Loop::kUpdate:
+      run_update<T>(h.set, h.args.data(), adt_a, save_a, bres_a, nb, keep_res);
+      break;
     case Loop::kUpdateBatch:
-      run_update_batch<T>(h.set, h.args.data(), adt_a, save_a, bres_a, nb);
+      run_update_batch<T>(h.set, h.args.data(), adt_a, save_a, bres_a, nb, keep_res);
       break;
     default: fail("op2: loop %s cannot be held for fusion", h.name.c_str());
   }
 }
 
-// Launches the held loop, fused with the adt_calc of adt_a if given.
-void release_held(const op_arg *adt_a) {
+// Launches the held loop, fused with the adt_calc of adt_a if given (or
+// the one held with it).  keep_res: the res_calc that follows overwrites
+// every cell's res without reading it, so update's zeros are not stored.
+void release_held(const op_arg *adt_a, bool keep_res = false) {
   Held &h = held();
   if (!h.active) return;
   h.active = false;
   flush_fold();
+  std::vector<op_arg> adt;
+  adt.swap(h.adt_args);
+  if (!adt_a && !adt.empty()) adt_a = adt.data();
   if (h.rb == 8)
-    launch_held<double>(h, adt_a);
+    launch_held<double>(h, adt_a, keep_res);
   else
-    launch_held<float>(h, adt_a);
+    launch_held<float>(h, adt_a, keep_res);
   check_launch(h.name.c_str());
   note_writes((int)h.bres_args.size(), h.bres_args.data());
   note_writes((int)h.args.size(), h.args.data());
@@ -2038,6 +2056,23 @@ void release_held(const op_arg *adt_a) {
   if (h.l == Loop::kUpdate || h.l == Loop::kUpdateBatch) zero_dats().insert(h.args[2].dat);
   h.save_args.clear();
   h.bres_args.clear();
+}
+
+// A res_calc call (l, rb, set, a) that follows a held update + adt_calc
+// and overwrites every cell of the res that update zeroes without reading
+// it: an owner-compute plan (the kernels that take res as known zero) whose
+// tiles own every cell.
+bool overwrites_held_res(const char *name, Loop l, int rb, op_set set, int nargs, op_arg *a) {
+  const Held &h = held();
+  if (!h.active || h.adt_args.empty() || l != Loop::kResCalc) return false;
+  if (h.l != Loop::kUpdate && h.l != Loop::kUpdateBatch) return false;
+  if (a[6].dat != h.args[2].dat || a[6].map->to != h.set) return false;
+  auto ot = rt().online.find(name);
+  if (ot != rt().online.end() && !ot->second.done) return false;
+  const int strategy = strategy_of(name, rb);
+  if (strategy != OP2_STRATEGY_OWNER && strategy != OP2_STRATEGY_OWNER_GATHER) return false;
+  const Plan *p = plan_for(name, set, nargs, a, strategy, part_of(name, 256));
+  return p->owned.size() == (size_t)h.set->size;
 }
 
 // An update call (l, rb, set, a) on the cells whose res a held bres_calc
@@ -2051,7 +2086,7 @@ bool takes_held_bres(Loop l, int rb, op_set set, const op_arg *a) {
 // A save_soln call (l, rb, set, a) that copies the q a held update writes.
 bool joins_held(Loop l, int rb, op_set set, const op_arg *a) {
   const Held &h = held();
-  return h.active && l == Loop::kSaveSoln && h.save_args.empty() &&
+  return h.active && l == Loop::kSaveSoln && h.save_args.empty() && h.adt_args.empty() &&
          (h.l == Loop::kUpdate || h.l == Loop::kUpdateBatch) && h.set == set && h.rb == rb &&
          a[0].dat == h.args[1].dat;
 }
@@ -2059,7 +2094,8 @@ bool joins_held(Loop l, int rb, op_set set, const op_arg *a) {
 // An adt_calc call (l, rb, set, a) that completes the held loop.
 bool fuses_with_held(Loop l, int rb, op_set set, const op_arg *a) {
   const Held &h = held();
-  if (!h.active || l != Loop::kAdtCalc || h.set != set || h.rb != rb) return false;
+  if (!h.active || !h.adt_args.empty() || l != Loop::kAdtCalc || h.set != set || h.rb != rb)
+    return false;
   const op_dat q = h.l == Loop::kSaveSoln ? h.args[0].dat : h.args[1].dat;
   return a[4].dat == q;
 }
@@ -2074,6 +2110,7 @@ void fusion_drop() {
   held().active = false;
   held().save_args.clear();
   held().bres_args.clear();
+  held().adt_args.clear();
   pending_fold().active = false;
   zero_dats().clear();
 }
@@ -2130,6 +2167,11 @@ int run_loop(const char *name, op_set set, int nargs, op_arg *args) {
   Runtime &r = rt();
   const bool fuse = r.capturing && r.graph_fusion;
   if (fuse && fuses_with_held(l, rb, set, args)) {
+    Held &h = held();
+    if (h.l == Loop::kUpdate || h.l == Loop::kUpdateBatch) {
+      h.adt_args.assign(args, args + nargs);  // held until the next call (keep_res)
+      return 0;
+    }
     release_held(args);
     return 0;
   }
@@ -2146,7 +2188,7 @@ int run_loop(const char *name, op_set set, int nargs, op_arg *args) {
     h.args.assign(args, args + nargs);
     return 0;
   }
-  release_held(nullptr);
+  release_held(nullptr, fuse && overwrites_held_res(name, l, rb, set, nargs, args));
   if (fuse && (l == Loop::kSaveSoln || l == Loop::kUpdate || l == Loop::kUpdateBatch ||
                (l == Loop::kBresCalc && find_bres_index(nargs, args) &&
                 find_bres_index(nargs, args)->usable))) {
