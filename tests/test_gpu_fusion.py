@@ -113,6 +113,24 @@ def test_fused_boundary_cells_with_two_boundary_edges(runtime, dtype):
         assert np.max(np.abs(h - oh) / oh) < 1e-4
 
 
+@pytest.mark.parametrize("renumber", [False, True])
+def test_cell_without_edges_keeps_zeroed_res(runtime, renumber):
+    """A cell that no edge touches is not owned by any res_calc tile, so the
+    update before res_calc must still store its zero res (res_calc will not
+    overwrite it): fused equals unfused, and both match the oracle."""
+    note(f"fused graph, 24x12 O-mesh plus a cell without edges, renumber={renumber}")
+    m = M.o_mesh(24, 12, "random", seed=10)
+    mesh = dataclasses.replace(m, cell=np.ascontiguousarray(np.concatenate([m.cell, m.cell[:1]])))
+    c = M.flow_constants().as_array()
+    h1, q1, a1, r1 = graph_run(mesh, c, "double", 13, True, renumber, 4)
+    op2.op_exit()
+    op2.op_init([], 0)
+    h0, q0, a0, r0 = graph_run(mesh, c, "double", 13, False, renumber, 4)
+    assert np.array_equal(h1, h0) and np.array_equal(q1, q0) and np.array_equal(r1, r0)
+    st, oh = O.airfoil_seq(mesh, c, 13, "f64")
+    assert np.max(np.abs(h1 - oh) / oh) < 1e-12
+
+
 def test_cell_with_three_boundary_edges_is_not_fused(runtime):
     """A boundary edge listed twice gives its cell three: that bres_calc
     keeps its own kernel in the graph, and the run still matches the oracle
