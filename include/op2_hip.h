@@ -241,6 +241,13 @@ int const *op2_map_device_ptr(op_map map);
  * call that asks for more). */
 void *op2_device_scratch(size_t bytes);
 
+/* With deferred reductions on (op2_set_deferred_reductions), the device
+ * reduction log, its entry counter and its capacity (a power of two);
+ * returns 1, else 0.  The generic path appends its OP_GBL results there, as
+ * the built-in loops do, so that its loops can be captured into graphs.  No
+ * counterpart in the reference. */
+int op2_reduction_log_device(double **log, unsigned **count, unsigned *capacity);
+
 /* Checkpoint and fast-forward restart (doc/dev.tex:1080-1087 describes
  * op_checkpoint; the reference does not implement it).
  * op2_checkpoint_record(path): from now on count loop calls and keep the

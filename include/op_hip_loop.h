@@ -221,6 +221,29 @@ __global__ __launch_bounds__(kThreads) void k_loop_fold(const double *partials, 
   if (threadIdx.x == 0) out[c] = v;
 }
 
+// Deferred reductions: one workgroup folds every component in turn, each
+// exactly as k_loop_fold does, and appends the results to the runtime's
+// device log in component order (a loop's sum for OP_INC; for OP_MIN/OP_MAX
+// the result including the caller's value at launch, as the caller would get).
+template <int D = 0>
+__global__ __launch_bounds__(kThreads) void k_loop_fold_log(const double *partials, int nslots,
+                                                            RedKinds rk, double *log,
+                                                            unsigned *count, unsigned mask) {
+  __shared__ double lds[kThreads / 64];
+  __shared__ unsigned base;
+  if (threadIdx.x == 0) base = atomicAdd(count, (unsigned)rk.n);
+  __syncthreads();
+  for (int c = 0; c < rk.n; c++) {
+    const int kind = rk.kind[c];
+    double v = red_identity(kind);
+    for (int s = threadIdx.x; s < nslots; s += kThreads)
+      v = red_op(kind, v, partials[(size_t)s * rk.n + c]);
+    v = block_red(kind, v, lds);
+    if (threadIdx.x == 0) log[(base + c) & mask] = v;
+    __syncthreads();
+  }
+}
+
 inline int error(char const *name, char const *what) {
   char buf[512];
   snprintf(buf, sizeof buf, "op_par_loop error -- loop \"%s\": %s", name, what);
@@ -353,6 +376,16 @@ int run(char const *name, op_set set, int nargs, op_arg *args, std::tuple<T...> 
                        partials);
   }
   if (hipGetLastError() != hipSuccess) return error(name, "kernel launch failed");
+  double *log = nullptr;
+  unsigned *log_count = nullptr, log_cap = 0;
+  if (rk.n > 0 && op2_reduction_log_device(&log, &log_count, &log_cap) == 1) {
+    // deferred: the results go to the device log, the caller's variables are
+    // left alone, nothing waits (an empty set logs each reduction's identity)
+    hipLaunchKernelGGL(k_loop_fold_log<0>, dim3(1), dim3(kThreads), 0, st,
+                       (const double *)partials, nslots, rk, log, log_count, log_cap - 1);
+    if (hipGetLastError() != hipSuccess) return error(name, "reduction fold launch failed");
+    return 0;
+  }
   if (rk.n > 0 && nslots > 0) {  // (no blocks: no element, nothing to combine)
     double *out = partials + (size_t)nslots * rk.n;
     hipLaunchKernelGGL(k_loop_fold<0>, dim3(rk.n), dim3(kThreads), 0, st,

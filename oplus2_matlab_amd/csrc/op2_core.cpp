@@ -847,6 +847,15 @@ int const *op2_map_device_ptr(op_map map) {
   API_END(nullptr)
 }
 
+int op2_reduction_log_device(double **log, unsigned **count, unsigned *capacity) {
+  Runtime &r = rt();
+  if (!r.deferred || !r.d_log) return 0;
+  if (log) *log = r.d_log;
+  if (count) *count = r.d_log_count;
+  if (capacity) *capacity = kLogCap;
+  return 1;
+}
+
 void *op2_device_scratch(size_t bytes) {
   API_BEGIN
   return reduct_scratch(bytes);

@@ -134,6 +134,7 @@ SIGNATURES = {
     "op2_graph_launch": (C.c_int, [C.c_int, C.c_int]),
     "op2_graph_destroy": (C.c_int, [C.c_int]),
     "op2_set_graph_fusion": (C.c_int, [C.c_int]),
+    "op2_reduction_log_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
 }
 
 _lib = None

@@ -233,3 +233,50 @@ def test_generic_res_calc_equals_the_device_kernel(runtime, renumber):
     op2.op_par_loop(None, "gen_res_calc", a.edges, *args)
     got = op2.op_fetch_data(pres2).reshape(-1, 4)
     assert np.max(np.abs(got - native)) <= 1e-13 * np.max(np.abs(native))
+
+
+def test_reductions_in_a_captured_graph(runtime):
+    """With deferred reductions a generic loop's OP_GBL results go to the
+    device log (the loop's sum for OP_INC, the result including the caller's
+    value for OP_MAX), so the loop can be captured into a graph and replayed."""
+    note("generic edge loop with OP_GBL INC and MAX captured in a graph, replayed")
+    mesh = M.o_mesh(48, 24, "random", seed=11)
+    nodes = op2.op_decl_set(mesh.nnode, "nodes")
+    edges = op2.op_decl_set(mesh.nedge, "edges")
+    pedge = op2.op_decl_map(edges, nodes, 2, mesh.edge, "pedge")
+    px = op2.op_decl_dat(nodes, 2, "double", np.ascontiguousarray(mesh.x), "px")
+    pc = op2.op_decl_dat(nodes, 1, "int", np.zeros(mesh.nnode, np.int32), "pc")
+    total = np.array([1.0])
+    longest = np.array([0.0])
+
+    def loop():
+        op2.op_par_loop(None, "gen_edge_sum", edges,
+                        op_arg_dat(px, 0, pedge, 2, "double", OP_READ),
+                        op_arg_dat(px, 1, pedge, 2, "double", OP_READ),
+                        op_arg_dat(pc, 0, pedge, 1, "int", OP_INC),
+                        op_arg_dat(pc, 1, pedge, 1, "int", OP_INC),
+                        op_arg_gbl(total, 1, "double", OP_INC),
+                        op_arg_gbl(longest, 1, "double", OP_MAX))
+
+    loop()  # eager: plan and maps on the device
+    e1, e2 = mesh.edge[:, 0], mesh.edge[:, 1]
+    ln = np.hypot(mesh.x[e2, 0] - mesh.x[e1, 0], mesh.x[e2, 1] - mesh.x[e1, 1])
+    np.testing.assert_allclose(total[0], 1.0 + ln.sum(), rtol=1e-13)
+    op2.set_deferred_reductions(True)
+    try:
+        op2.reduction_log()
+        op2.graph_begin()
+        loop()
+        g = op2.graph_end()
+        op2.graph_launch(g, 2)
+        loop()  # eager, deferred, between replays
+        op2.graph_launch(g, 1)
+        log = op2.reduction_log()
+    finally:
+        op2.set_deferred_reductions(False)
+    assert log.size == 8
+    np.testing.assert_allclose(log[0::2], ln.sum(), rtol=1e-13)
+    assert np.all(log[1::2] == ln.max())
+    assert total[0] == 1.0 + ln.sum() or np.isclose(total[0], 1.0 + ln.sum(), rtol=1e-13)
+    want_cnt = 5 * (np.bincount(e1, minlength=mesh.nnode) + np.bincount(e2, minlength=mesh.nnode))
+    assert np.array_equal(op2.op_fetch_data(pc), want_cnt.astype(np.int32))
