@@ -7,7 +7,7 @@ the 2400 x 1200 O-mesh (2.88M cells, 5.76M edges), with every dat resident
 in HBM.  The iteration is captured once as a hipGraph with deferred
 reductions (the rms of every iteration lands in the runtime's device log)
 and replayed; value = cells x steps / seconds of the timed replays (FP64).
-Each graph holds --graph-iters iterations (default 8), and the capture
+Each graph holds --graph-iters iterations (default 10), and the capture
 fuses loops that can share a pass: update with the next iteration's
 save_soln and each update or save_soln with the adt_calc that follows
 (--no-fusion captures the loops one kernel each).
@@ -87,7 +87,7 @@ def parse():
                     help="target CPU time of the cpu_baseline sample (0 = skip)")
     ap.add_argument("--no-fusion", dest="fusion", action="store_false",
                     help="capture the graphs without loop fusion")
-    ap.add_argument("--graph-iters", type=int, default=8,
+    ap.add_argument("--graph-iters", type=int, default=10,
                     help="iterations per captured graph (its gcd with the step count)")
     return ap.parse_args()
 
