@@ -1,0 +1,44 @@
+"""Parity at the benchmark's full size: the 2400 x 1200 O-mesh (2.88M
+cells, 5.76M edges) in FP64, 1000 iterations exactly as bench.py times them
+(renumbered, owner-gather res_calc with 256-cell tiles, iterations replayed
+as fused hipGraphs of 10), against the oracle's OpenMP schedule of the
+reference on the host (itself within ~1e-13 of the sequential order)."""
+
+import os
+
+import numpy as np
+import pytest
+
+from oplus2_matlab_amd import airfoil as AF
+from oplus2_matlab_amd import mesh as M
+from oplus2_matlab_amd import op2
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def note(msg):
+    print(f"[gpu] {msg}", flush=True)
+
+
+def test_airfoil_2400x1200_1000_iterations_fused_graphs():
+    note("airfoil FP64 2400x1200, 1000 iterations in fused graphs of 10, vs the OpenMP oracle")
+    mesh = M.o_mesh(2400, 1200, "cartesian", seed=1)
+    c = M.flow_constants().as_array()
+    op2.op_init([], 0)
+    try:
+        for loop in ("res_calc", "bres_calc"):
+            op2.set_loop_config(loop, op2.STRATEGY_OWNER_GATHER, 256, 0)
+        hist, q, _, res = AF.run_airfoil(mesh, c, 1000, "double", graph=True, renumber=True,
+                                         graph_iters=10)
+    finally:
+        op2.op_exit()
+        op2.set_loop_config(None, 0, 0, 0)
+    note("GPU done; oracle OpenMP run")
+    nthr = max(1, min(16, os.cpu_count() or 1))
+    st, ohist, _ = O.airfoil_omp(mesh, c, 1000, "f64", nthreads=nthr)
+    rel = np.max(np.abs(hist - ohist) / np.abs(ohist))
+    relq = np.max(np.abs(q - st.q.reshape(-1, 4))) / np.max(np.abs(st.q))
+    note(f"rms history max rel {rel:.2e}, q max rel {relq:.2e}")
+    assert rel < 1e-12 and relq < 1e-12
+    assert not res.any()
