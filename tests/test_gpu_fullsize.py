@@ -42,3 +42,31 @@ def test_airfoil_2400x1200_1000_iterations_fused_graphs():
     note(f"rms history max rel {rel:.2e}, q max rel {relq:.2e}")
     assert rel < 1e-12 and relq < 1e-12
     assert not res.any()
+
+
+def test_batch_64_meshes_fused_graphs():
+    """The benchmark's batch: 64 meshes of 10k-400k cells (7.12M cells) with
+    their own free streams, renumbered, 100 iterations in fused graphs of 10;
+    every mesh against the OpenMP oracle run of that mesh alone."""
+    note("batch FP64, 64 meshes / 7.12M cells, 100 iterations in fused graphs vs the oracle")
+    batch = M.mesh_batch(nmesh=64, seed=1)
+    op2.op_init([], 0)
+    try:
+        for loop in ("res_calc", "bres_calc"):
+            op2.set_loop_config(loop, op2.STRATEGY_OWNER_GATHER, 256, 0)
+        hist, q = AF.run_airfoil_batch(batch, 100, "double", graph=True, renumber=True,
+                                       graph_iters=10)
+    finally:
+        op2.op_exit()
+        op2.set_loop_config(None, 0, 0, 0)
+    note("GPU done; oracle OpenMP runs")
+    nthr = max(1, min(16, os.cpu_count() or 1))
+    worst = worst_q = 0.0
+    for i in range(batch.nmesh):
+        st, oh, _ = O.airfoil_omp(batch.meshes[i], batch.consts[i], 100, "f64", nthreads=nthr)
+        qi = q[batch.cell_off[i]:batch.cell_off[i + 1]]
+        worst = max(worst, float(np.max(np.abs(hist[:, i] - oh) / np.abs(oh))))
+        worst_q = max(worst_q,
+                      float(np.max(np.abs(qi - st.q.reshape(-1, 4))) / np.max(np.abs(st.q))))
+    note(f"rms history max rel {worst:.2e}, q max rel {worst_q:.2e} over 64 meshes")
+    assert worst < 1e-12 and worst_q < 1e-12
