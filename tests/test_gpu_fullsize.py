@@ -1,8 +1,10 @@
-"""Parity at the benchmark's full size: the 2400 x 1200 O-mesh (2.88M
-cells, 5.76M edges) in FP64, 1000 iterations exactly as bench.py times them
-(renumbered, owner-gather res_calc with 256-cell tiles, iterations replayed
-as fused hipGraphs of 10), against the oracle's OpenMP schedule of the
-reference on the host (itself within ~1e-13 of the sequential order)."""
+"""Parity at the benchmark's full sizes: the 2400 x 1200 O-mesh (2.88M
+cells, 5.76M edges) in FP64 and the FP32 form, 1000 iterations as bench.py
+runs them (renumbered, owner-gather res_calc with 256-cell tiles,
+iterations replayed as fused hipGraphs of 10), the same mesh in a
+seed-random ordering, and the 64-mesh batch, against the oracle's OpenMP
+schedule of the reference on the host (itself within ~1e-13 of the
+sequential order)."""
 
 import os
 
@@ -21,27 +23,75 @@ def note(msg):
     print(f"[gpu] {msg}", flush=True)
 
 
-def test_airfoil_2400x1200_1000_iterations_fused_graphs():
-    note("airfoil FP64 2400x1200, 1000 iterations in fused graphs of 10, vs the OpenMP oracle")
-    mesh = M.o_mesh(2400, 1200, "cartesian", seed=1)
-    c = M.flow_constants().as_array()
+NTHR = max(1, min(16, os.cpu_count() or 1))
+
+
+def gpu_run(mesh, c, niter, dtype, renumber=True, strategy=op2.STRATEGY_OWNER_GATHER, part=256):
     op2.op_init([], 0)
     try:
         for loop in ("res_calc", "bres_calc"):
-            op2.set_loop_config(loop, op2.STRATEGY_OWNER_GATHER, 256, 0)
-        hist, q, _, res = AF.run_airfoil(mesh, c, 1000, "double", graph=True, renumber=True,
-                                         graph_iters=10)
+            op2.set_loop_config(loop, strategy, part, 0)
+        return AF.run_airfoil(mesh, c, niter, dtype, graph=True, renumber=renumber,
+                              graph_iters=10)
     finally:
         op2.op_exit()
         op2.set_loop_config(None, 0, 0, 0)
-    note("GPU done; oracle OpenMP run")
-    nthr = max(1, min(16, os.cpu_count() or 1))
-    st, ohist, _ = O.airfoil_omp(mesh, c, 1000, "f64", nthreads=nthr)
+
+
+@pytest.fixture(scope="module")
+def full():
+    """The 2400 x 1200 mesh and the oracle's OpenMP FP64 run of 1000
+    iterations on it (shared by the FP64 and FP32-form tests)."""
+    mesh = M.o_mesh(2400, 1200, "cartesian", seed=1)
+    c = M.flow_constants().as_array()
+    note("oracle OpenMP FP64 run, 2400x1200, 1000 iterations")
+    st, ohist, _ = O.airfoil_omp(mesh, c, 1000, "f64", nthreads=NTHR)
+    return mesh, c, st.q.reshape(-1, 4).copy(), ohist
+
+
+def test_airfoil_2400x1200_1000_iterations_fused_graphs(full):
+    mesh, c, oq, ohist = full
+    note("airfoil FP64 2400x1200, 1000 iterations in fused graphs of 10, vs the OpenMP oracle")
+    hist, q, _, res = gpu_run(mesh, c, 1000, "double")
     rel = np.max(np.abs(hist - ohist) / np.abs(ohist))
-    relq = np.max(np.abs(q - st.q.reshape(-1, 4))) / np.max(np.abs(st.q))
+    relq = np.max(np.abs(q - oq)) / np.max(np.abs(oq))
     note(f"rms history max rel {rel:.2e}, q max rel {relq:.2e}")
     assert rel < 1e-12 and relq < 1e-12
     assert not res.any()
+
+
+def test_fp32_form_2400x1200_error_bound(full):
+    """The FP32 form (float state and flux, double update and rms) stays
+    within 1e-5 relative of the FP64 rms history after 1000 iterations."""
+    mesh, c, oq, ohist = full
+    note("airfoil FP32 form 2400x1200, 1000 iterations in fused graphs, vs the FP64 oracle")
+    hist, q, _, _ = gpu_run(mesh, c, 1000, "float")
+    rel = np.max(np.abs(hist - ohist) / np.abs(ohist))
+    dq = float(np.max(np.abs(q.astype(np.float64) - oq)))
+    note(f"rms history max rel {rel:.2e}, max |q - q64| {dq:.2e}")
+    assert rel < 1e-5
+    assert dq < 1e-3
+
+
+@pytest.mark.parametrize("renumber", [True, False])
+def test_random_ordering_2400x1200(renumber):
+    """The same mesh with every set numbered by a seed-random permutation,
+    renumbered by the runtime (owner-gather, 256-cell tiles) or as given
+    (owner-compute, 128-cell tiles), 100 iterations in fused graphs."""
+    note(f"airfoil FP64 2400x1200 random ordering, renumber={renumber}, 100 iterations")
+    mesh = M.o_mesh(2400, 1200, "random", seed=1)
+    c = M.flow_constants().as_array()
+    if renumber:
+        hist, q, _, _ = gpu_run(mesh, c, 100, "double")
+    else:
+        hist, q, _, _ = gpu_run(mesh, c, 100, "double", renumber=False,
+                                strategy=op2.STRATEGY_OWNER, part=128)
+    st, ohist, _ = O.airfoil_omp(mesh, c, 100, "f64", nthreads=NTHR)
+    oq = st.q.reshape(-1, 4)
+    rel = np.max(np.abs(hist - ohist) / np.abs(ohist))
+    relq = np.max(np.abs(q - oq)) / np.max(np.abs(oq))
+    note(f"rms history max rel {rel:.2e}, q max rel {relq:.2e}")
+    assert rel < 1e-12 and relq < 1e-12
 
 
 def test_batch_64_meshes_fused_graphs():
@@ -60,10 +110,9 @@ def test_batch_64_meshes_fused_graphs():
         op2.op_exit()
         op2.set_loop_config(None, 0, 0, 0)
     note("GPU done; oracle OpenMP runs")
-    nthr = max(1, min(16, os.cpu_count() or 1))
     worst = worst_q = 0.0
     for i in range(batch.nmesh):
-        st, oh, _ = O.airfoil_omp(batch.meshes[i], batch.consts[i], 100, "f64", nthreads=nthr)
+        st, oh, _ = O.airfoil_omp(batch.meshes[i], batch.consts[i], 100, "f64", nthreads=NTHR)
         qi = q[batch.cell_off[i]:batch.cell_off[i + 1]]
         worst = max(worst, float(np.max(np.abs(hist[:, i] - oh) / np.abs(oh))))
         worst_q = max(worst_q,
