@@ -212,8 +212,9 @@ def measure_batch(a, batch, dtype):
             "tuned": tuned, "loops": loops}
 
 
-def accuracy(mesh, consts, iters, renumber):
-    """FP32 form against FP64 after `iters` iterations (graph replay)."""
+def accuracy(mesh, consts, iters, renumber, graph_iters=1):
+    """FP32 form against FP64 after `iters` iterations, replayed as graphs of
+    `graph_iters` fused iterations (the path the throughput figures time)."""
     import numpy as np
 
     from oplus2_matlab_amd import airfoil as AF
@@ -224,12 +225,13 @@ def accuracy(mesh, consts, iters, renumber):
         op2.op_init([], 0)
         try:
             out[dtype] = AF.run_airfoil(mesh, consts, iters, dtype, graph=True,
-                                        renumber=renumber)
+                                        renumber=renumber,
+                                        graph_iters=math.gcd(max(graph_iters, 1), iters))
         finally:
             op2.op_exit()
     h64, q64 = out["double"][0], out["double"][1]
     h32, q32 = out["float"][0], out["float"][1]
-    return {"iterations": iters,
+    return {"iterations": iters, "graph_iters": math.gcd(max(graph_iters, 1), iters),
             "rms_history_max_rel_err": float(np.max(np.abs(h32 - h64) / np.abs(h64))),
             "rms_final_rel_err": float(abs(h32[-1] - h64[-1]) / abs(h64[-1])),
             "max_abs_q_minus_q64": float(np.max(np.abs(q32.astype(np.float64) - q64)))}
@@ -319,7 +321,7 @@ def main():
                                "loops": r32["loops"]}
         if a.accuracy_iters > 0:
             result["fp32_form"]["accuracy"] = accuracy(mesh, consts, a.accuracy_iters,
-                                                       a.renumber == "on")
+                                                       a.renumber == "on", a.graph_iters)
     if a.small_mesh:
         # the 600 x 300 O-mesh (180k cells) with seed-random numbering
         ms = M.o_mesh(600, 300, "random", seed=1)
