@@ -298,10 +298,16 @@ int op2_graph_end(void);
 int op2_graph_launch(int id, int count);
 int op2_graph_destroy(int id);
 
-/* Loop fusion during graph capture (on by default): a captured save_soln or
- * update followed by an adt_calc on the same cells that reads the q they
- * write runs as one kernel (q is read once), with the same results bit for
- * bit.  Returns the previous setting, or -1 during a capture (no change). */
+/* Loop fusion during graph capture (on by default).  A captured bres_calc
+ * joins the update of the cells whose res it increments; an update joins the
+ * save_soln that copies its q and the adt_calc that reads it, so q is read
+ * once; a save_soln alone joins that adt_calc; the rms sum of an update is
+ * folded by the next res_calc's kernel; and a res that an earlier captured
+ * update zeroed is taken as zero (not read, and not stored when the next
+ * owner-compute res_calc overwrites every cell).  Results are the same bit
+ * for bit.  Fusion sees only op_par_loop calls: other work captured on
+ * op2_stream that writes a dat calls for fusion off during that capture.
+ * Returns the previous setting, or -1 during a capture (no change). */
 int op2_set_graph_fusion(int enable);
 
 /* Device pointer of a dat (for zero-copy interop). */
