@@ -190,7 +190,10 @@ int op2_autotune_online_result(char const *name, int *best_strategy, int *best_p
  * are permuted; the user's arrays keep the user's numbering, op_fetch_data
  * returns data in it, and dats and maps declared later are translated.
  * Plans (and op_plan_get views) use the internal numbering.  Call before
- * capturing graphs; at most once per op_init.  Returns 0. */
+ * capturing graphs; at most once per op_init.  Returns 0.
+ * Programs that do not call it get the same with OP2_RENUMBER=auto (or 1,
+ * on) in the environment at op_init: the runtime renumbers at the first
+ * loop with an indirect OP_INC argument, on that argument's map. */
 int op2_renumber(op_map base);
 
 /* The internal number of every element of `set` (new_of_old[set size]; may

@@ -5,6 +5,7 @@
 
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <algorithm>
 #include <cstring>
 
@@ -255,6 +256,7 @@ static void clear_all() {
   r.timings.clear();
   r.consts.clear();
   r.set_perm.clear();
+  r.auto_renumber = false;
   r.set_segments.clear();
   r.map_internal.clear();
   checkpoint_reset();
@@ -314,6 +316,10 @@ void op_init_core(int argc, char **argv, int diags) {
       printf("\n OP_cache_line_size  = %d \n", OP_cache_line_size);
     }
   }
+  // OP2_RENUMBER=1 (or auto, on): renumber without a call in the program
+  const char *rn = getenv("OP2_RENUMBER");
+  rt().auto_renumber =
+      rn && (!strcmp(rn, "1") || !strcmp(rn, "auto") || !strcmp(rn, "on"));
   rt().initialised = true;
   API_END()
 }

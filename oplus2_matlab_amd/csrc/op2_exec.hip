@@ -2127,6 +2127,7 @@ namespace {
 
 int run_loop(const char *name, op_set set, int nargs, op_arg *args) {
   ensure_device();
+  if (rt().auto_renumber) auto_renumber(set, nargs, args);
   if (OP_diags > 0) {
     int ninds = 0;
     for (int m = 0; m < nargs; m++) op_arg_check(set, m, args[m], &ninds, name);

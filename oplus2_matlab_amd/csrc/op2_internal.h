@@ -184,6 +184,7 @@ struct Runtime {
   // set, the starts of its independently ordered pieces, and the
   // renumbered host copies of the maps (op_map::map_i points into them)
   std::map<op_set, std::vector<int>> set_perm;
+  bool auto_renumber = false;  // OP2_RENUMBER set at op_init, not yet applied
   std::map<op_set, std::vector<int>> set_segments;
   std::map<op_map, std::vector<int>> map_internal;
   // generic loop path: loops and device constants registered by the
@@ -217,6 +218,9 @@ void checkpoint_reset();
 inline const int *map_host(op_map m) { return m->map_i ? m->map_i : m->map; }
 const std::vector<int> *set_new_of_old(op_set s);
 void renumber(op_map base);
+// with OP2_RENUMBER set: renumbers once, on the map of the first loop with
+// an indirect OP_INC argument (for programs that do not call op2_renumber)
+void auto_renumber(op_set set, int nargs, const op_arg *args);
 void renumber_map(op_map m);  // (re)builds m's internal copy for the current numbering
 void permute_to_internal(op_dat d, const char *user, char *internal);
 void permute_to_user(op_dat d, const char *internal, char *user);

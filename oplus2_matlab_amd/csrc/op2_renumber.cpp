@@ -361,4 +361,29 @@ void renumber(op_map base) {
            "dats %.2f s\n", t1 - t0, t2 - t1, t3 - t2, t4 - t3, now() - t4);
 }
 
+// The first loop with an indirect OP_INC argument names the mesh's main
+// connectivity (airfoil's res_calc: edges to cells; jac: edges to nodes), so
+// its map is the base.  Loops before it ran on the declared numbering; their
+// plans are dropped with the others.  Nothing happens once graphs exist or
+// the program has renumbered itself.
+void auto_renumber(op_set set, int nargs, const op_arg *args) {
+  Runtime &r = rt();
+  if (r.capturing || !r.graphs.empty() || !r.set_perm.empty()) {
+    r.auto_renumber = false;
+    return;
+  }
+  for (auto &o : r.online)
+    if (o.second.started && !o.second.done) return;  // after the tuning
+  for (int m = 0; m < nargs; m++) {
+    const op_arg &a = args[m];
+    if (a.argtype != OP_ARG_DAT || a.acc != OP_INC || !a.map || a.map->from != set ||
+        a.map->to == set)
+      continue;
+    r.auto_renumber = false;
+    if (OP_diags > 1) printf(" renumbering on map %s (OP2_RENUMBER)\n", a.map->name);
+    renumber(a.map);
+    return;
+  }
+}
+
 }  // namespace op2

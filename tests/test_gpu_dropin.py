@@ -31,9 +31,10 @@ def need(name):
     return path
 
 
-def run(path, cwd):
-    print(f"[gpu] running {os.path.basename(path)}", flush=True)
-    out = subprocess.run([path], cwd=cwd, capture_output=True, text=True, timeout=180)
+def run(path, cwd, env=None):
+    print(f"[gpu] running {os.path.basename(path)} {env or ''}", flush=True)
+    full = dict(os.environ, **env) if env else None
+    out = subprocess.run([path], cwd=cwd, capture_output=True, text=True, timeout=180, env=full)
     assert out.returncode == 0, out.stdout[-2000:] + out.stderr[-2000:]
     return out.stdout
 
@@ -52,6 +53,22 @@ def test_unchanged_airfoil_program(tmp_path, golden_dir, prog):
     exe = need(prog)
     M.write_grid_dat(M.o_mesh(120, 60, "cartesian"), os.path.join(tmp_path, "new_grid.dat"))
     got = rms_lines(run(exe, tmp_path))
+    ref = rms_lines(open(os.path.join(golden_dir, "airfoil_ref_120x60_cartesian.txt")).read())
+    assert sorted(got) == sorted(ref) == list(range(100, 1001, 100))
+    for it in ref:
+        assert abs(got[it] - ref[it]) <= 2e-4 * abs(ref[it]), (it, got[it], ref[it])
+
+
+@pytest.mark.parametrize("prog", ["airfoil", "airfoil_op"])
+def test_unchanged_airfoil_renumbered_by_the_environment(tmp_path, golden_dir, prog):
+    """OP2_RENUMBER=auto: the same mesh written in a random numbering is
+    renumbered by the runtime at the program's first res_calc (no call in
+    the program) and gives the golden rms history of the cartesian one."""
+    exe = need(prog)
+    M.write_grid_dat(M.o_mesh(120, 60, "random", seed=3), os.path.join(tmp_path, "new_grid.dat"))
+    out = run(exe, tmp_path, env={"OP2_RENUMBER": "auto"})
+    assert "renumbering on map pecell (OP2_RENUMBER)" in out
+    got = rms_lines(out)
     ref = rms_lines(open(os.path.join(golden_dir, "airfoil_ref_120x60_cartesian.txt")).read())
     assert sorted(got) == sorted(ref) == list(range(100, 1001, 100))
     for it in ref:
