@@ -131,7 +131,9 @@ int op2_set_error_mode(int mode);
 char const *op2_last_error(void);
 void op2_clear_error(void);
 
-/* Execution strategies for indirect loops with OP_INC arguments. */
+/* Execution strategies for indirect loops with OP_INC arguments.  AUTO (no
+ * configuration) is OWNER_GATHER, i.e. OWNER for loops without a gathering
+ * kernel. */
 #define OP2_STRATEGY_AUTO 0
 /* block colours (one launch per colour) and thread colours inside a block:
  * conflict-free, deterministic */

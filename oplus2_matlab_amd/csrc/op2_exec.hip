@@ -1122,13 +1122,15 @@ void check_launch(const char *loop) {
   if (e != hipSuccess) fail("op2: kernel launch for loop %s failed: %s", loop, hipGetErrorString(e));
 }
 
-// Default strategy when none is configured: FP64 increments go through
-// LDS and global atomics (one launch); FP32 through block and thread colours
-// (float global atomics are slow on this part).
+// Default strategy when none is configured: owner-compute tiles, gathering
+// from global memory where the loop has that kernel (res_calc), staged
+// otherwise: one launch, no atomics, deterministic, and the fastest
+// strategy at every mesh size measured, in both precisions.
 int strategy_of(const std::string &loop, int real_bytes) {
+  (void)real_bytes;
   int s = config_for(loop).strategy;
   if (s) return s;
-  return real_bytes == 8 ? OP2_STRATEGY_ATOMIC : OP2_STRATEGY_COLOURED;
+  return OP2_STRATEGY_OWNER_GATHER;
 }
 
 int part_of(const std::string &loop, int dflt) {
