@@ -1126,8 +1126,7 @@ void check_launch(const char *loop) {
 // from global memory where the loop has that kernel (res_calc), staged
 // otherwise: one launch, no atomics, deterministic, and the fastest
 // strategy at every mesh size measured, in both precisions.
-int strategy_of(const std::string &loop, int real_bytes) {
-  (void)real_bytes;
+int strategy_of(const std::string &loop) {
   int s = config_for(loop).strategy;
   if (s) return s;
   return OP2_STRATEGY_OWNER_GATHER;
@@ -1318,7 +1317,7 @@ void allow_lds(K kern, size_t lds) {
 
 template <typename T>
 void run_res_calc(const char *name, op_set set, int nargs, op_arg *a) {
-  const int strategy = strategy_of(name, sizeof(T));
+  const int strategy = strategy_of(name);
   if (strategy == OP2_STRATEGY_OWNER) {
     Plan *p = plan_for(name, set, nargs, a, strategy, part_of(name, 256));
     const PlanGroup &nodes = p->groups[p->slots[0].first];
@@ -1487,7 +1486,7 @@ BresIdx<T> bres_idx(const op_arg *b, int nb, bool keep_res) {
 template <typename T>
 void run_bres_calc(const char *name, op_set set, int nargs, op_arg *a) {
   if (rt().graph_fusion) build_bres_index(nargs, a);  // for captures that fuse it
-  const int requested = strategy_of(name, sizeof(T));
+  const int requested = strategy_of(name);
   const int strategy = narrow_strategy(requested);
   // batched form: the free stream of each boundary edge's mesh, through a map
   const T *qd = nargs == 7 ? (const T *)a[6].dat->data_d : nullptr;
@@ -1723,7 +1722,7 @@ void run_update_batch(op_set set, op_arg *a, const op_arg *adt_a = nullptr,
 
 template <typename T>
 void run_jac_res(const char *name, op_set set, int nargs, op_arg *a) {
-  const int strategy = narrow_strategy(strategy_of(name, sizeof(T)));
+  const int strategy = narrow_strategy(strategy_of(name));
   if (strategy == OP2_STRATEGY_OWNER) {
     Plan *p = plan_for(name, set, nargs, a, strategy, part_of(name, NT));
     if (p->arg_slot[1] != 0 || p->arg_slot[2] != 1) fail("op2: unexpected slot layout for %s", name);
@@ -2060,18 +2059,18 @@ void release_held(const op_arg *adt_a, bool keep_res = false) {
   h.bres_args.clear();
 }
 
-// A res_calc call (l, rb, set, a) that follows a held update + adt_calc
+// A res_calc call (l, set, a) that follows a held update + adt_calc
 // and overwrites every cell of the res that update zeroes without reading
 // it: an owner-compute plan (the kernels that take res as known zero) whose
 // tiles own every cell.
-bool overwrites_held_res(const char *name, Loop l, int rb, op_set set, int nargs, op_arg *a) {
+bool overwrites_held_res(const char *name, Loop l, op_set set, int nargs, op_arg *a) {
   const Held &h = held();
   if (!h.active || h.adt_args.empty() || l != Loop::kResCalc) return false;
   if (h.l != Loop::kUpdate && h.l != Loop::kUpdateBatch) return false;
   if (a[6].dat != h.args[2].dat || a[6].map->to != h.set) return false;
   auto ot = rt().online.find(name);
   if (ot != rt().online.end() && !ot->second.done) return false;
-  const int strategy = strategy_of(name, rb);
+  const int strategy = strategy_of(name);
   if (strategy != OP2_STRATEGY_OWNER && strategy != OP2_STRATEGY_OWNER_GATHER) return false;
   const Plan *p = plan_for(name, set, nargs, a, strategy, part_of(name, 256));
   return p->owned.size() == (size_t)h.set->size;
@@ -2191,7 +2190,7 @@ int run_loop(const char *name, op_set set, int nargs, op_arg *args) {
     h.args.assign(args, args + nargs);
     return 0;
   }
-  release_held(nullptr, fuse && overwrites_held_res(name, l, rb, set, nargs, args));
+  release_held(nullptr, fuse && overwrites_held_res(name, l, set, nargs, args));
   if (fuse && (l == Loop::kSaveSoln || l == Loop::kUpdate || l == Loop::kUpdateBatch ||
                (l == Loop::kBresCalc && find_bres_index(nargs, args) &&
                 find_bres_index(nargs, args)->usable))) {
