@@ -140,6 +140,31 @@ def test_airfoil_fp32_form(runtime):
     assert dq < 1e-4
 
 
+@pytest.mark.parametrize("dtype", ["double", "float"])
+def test_default_strategy_is_owner_gather(runtime, dtype):
+    """With no loop configuration res_calc and bres_calc run owner-compute
+    tiles: two runs give the same bits (no atomics), the same bits as
+    owner-gather with 256-cell tiles configured explicitly, and FP64 matches
+    the oracle."""
+    note(f"airfoil {dtype} 60x30 random, 30 iterations: default strategy twice, owner-gather")
+    mesh = M.o_mesh(60, 30, "random", seed=5)
+    c = M.flow_constants().as_array()
+    runs = []
+    for explicit in (False, False, True):
+        op2.op_exit()
+        op2.op_init([], 0)
+        if explicit:
+            configure(op2.STRATEGY_OWNER_GATHER, 256)
+        runs.append(AF.run_airfoil(mesh, c, 30, dtype))
+    for other in runs[1:]:
+        for a, b in zip(runs[0], other):
+            assert np.array_equal(a, b)
+    if dtype == "double":
+        st, h64 = O.airfoil_seq(mesh, c, 30, "f64")
+        assert relmax(runs[0][0], h64) < TOL64
+        assert relmax(runs[0][1], st.q) < TOL64
+
+
 # ---------------------------------------------------------------------------
 # Jacobi
 
