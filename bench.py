@@ -7,7 +7,8 @@ the 2400 x 1200 O-mesh (2.88M cells, 5.76M edges), with every dat resident
 in HBM.  The iteration is captured once as a hipGraph with deferred
 reductions (the rms of every iteration lands in the runtime's device log)
 and replayed; value = cells x steps / seconds of the timed replays (FP64).
-Each graph holds --graph-iters iterations (default 10), and the capture
+Each graph holds --graph-iters iterations (default 50: every graph boundary
+costs a separate save_soln + adt_calc kernel and one stored res), and the capture
 fuses loops that can share a pass: update with the next iteration's
 save_soln and each update or save_soln with the adt_calc that follows
 (--no-fusion captures the loops one kernel each).
@@ -87,7 +88,7 @@ def parse():
                     help="target CPU time of the cpu_baseline sample (0 = skip)")
     ap.add_argument("--no-fusion", dest="fusion", action="store_false",
                     help="capture the graphs without loop fusion")
-    ap.add_argument("--graph-iters", type=int, default=10,
+    ap.add_argument("--graph-iters", type=int, default=50,
                     help="iterations per captured graph (its gcd with the step count)")
     return ap.parse_args()
 
@@ -243,9 +244,10 @@ def cpu_baseline(mesh, consts, target_s):
     from oracle import oracle
 
     nthr = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
-    # one iteration sizes the sample; loop_seconds excludes plan construction
-    _, _, secs = oracle.airfoil_omp(mesh, consts, 1, "f64", nthreads=nthr)
-    per_iter = max(float(np.sum(secs)), 1e-3)
+    # five iterations size the sample (the first, cold one alone costs about
+    # two later ones); loop_seconds excludes plan construction
+    _, _, secs = oracle.airfoil_omp(mesh, consts, 5, "f64", nthreads=nthr)
+    per_iter = max(float(np.sum(secs)) / 5, 1e-3)
     iters = int(max(2, min(2000, target_s / per_iter)))
     _, _, secs = oracle.airfoil_omp(mesh, consts, iters, "f64", nthreads=nthr)
     t = float(np.sum(secs))
