@@ -449,9 +449,10 @@ def graph_destroy(gid: int) -> None:
 
 
 def set_graph_fusion(on: bool) -> bool:
-    """Loop fusion during graph capture (default on): save_soln or update
-    followed by an adt_calc on the q they write run as one kernel.  Returns
-    the previous setting."""
+    """Loop fusion during graph capture (default on): bres_calc, update, the
+    next save_soln and adt_calc run as one kernel, the rms fold rides in the
+    next res_calc, and a res that a captured update zeroed is taken as zero;
+    the results are the same bit for bit.  Returns the previous setting."""
     prev = _call("op2_set_graph_fusion", int(on))
     if prev < 0:
         raise OP2Error("op2_set_graph_fusion: not during graph capture")
