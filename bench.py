@@ -48,10 +48,11 @@ from __future__ import annotations
 
 import argparse
 import json
-import math
 import os
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -89,7 +90,8 @@ def parse():
     ap.add_argument("--no-fusion", dest="fusion", action="store_false",
                     help="capture the graphs without loop fusion")
     ap.add_argument("--graph-iters", type=int, default=50,
-                    help="iterations per captured graph (its gcd with the step count)")
+                    help="iterations per captured graph (at most the step count; a second "
+                         "graph runs the remainder)")
     return ap.parse_args()
 
 
@@ -110,6 +112,17 @@ def measured_traffic(ni, nj, res_cfg):
         if line.startswith("| `op2::k_res_calc_owner_direct<double"):
             return float(line.split("|")[7]) * 1e6
     return None
+
+
+def capture_graphs(af, graph_iters, nsteps):
+    """Graphs that run nsteps iterations: one of k = min(graph_iters, nsteps)
+    iterations (replayed nsteps // k times) and, when k does not divide
+    nsteps, one of the remaining nsteps % k iterations, both captured before
+    any timing.  Returns (k, graph id, remainder graph id or None)."""
+    k = max(1, min(graph_iters, nsteps))
+    gid = af.capture_iteration(k)
+    rid = af.capture_iteration(nsteps % k) if nsteps % k else None
+    return k, gid, rid
 
 
 def measure(mesh, consts, dtype, a, renumber=None, fixed=None, steps=None):
@@ -148,12 +161,13 @@ def measure(mesh, consts, dtype, a, renumber=None, fixed=None, steps=None):
                 n, secs, bytes_ = st
                 loops[name] = {"calls": n, "us_per_call": 1e6 * secs / n,
                                "GBs": bytes_ * n / secs / 1e9, "bytes_per_call": bytes_}
-        k = math.gcd(max(a.graph_iters, 1), nsteps)
-        gid = af.capture_iteration(k)
+        k, gid, rid = capture_graphs(af, a.graph_iters, nsteps)
         af.run_graph(gid, a.warmup)
         op2.sync()
         t0 = time.perf_counter()
         op2.graph_launch(gid, nsteps // k)
+        if rid is not None:
+            op2.graph_launch(rid, 1)
         op2.sync()
         dt_graph = time.perf_counter() - t0
         log = op2.reduction_log()
@@ -192,13 +206,26 @@ def measure_batch(a, batch, dtype):
             st = op2.loop_stats(name)
             if st:
                 loops[name] = {"calls": st[0], "us_per_call": 1e6 * st[1] / st[0]}
-        gid = af.capture_iteration(math.gcd(max(a.graph_iters, 1), a.steps))
+        k, gid, rid = capture_graphs(af, a.graph_iters, a.steps)
         af.run_graph(gid, a.warmup)
         op2.sync()
+        op2.reduction_log()
+        # the reduction log holds 2 x 64 results per iteration: replays are
+        # read back every `per` graphs (a sync) so that it never overflows
+        per = max(1, op2.REDUCTION_LOG_CAPACITY // (2 * batch.nmesh * k))
         t0 = time.perf_counter()
-        hist = af.run_graph(gid, a.steps)
+        done = 0
+        while done < a.steps // k:
+            n = min(per, a.steps // k - done)
+            op2.graph_launch(gid, n)
+            log = op2.reduction_log()
+            done += n
+        if rid is not None:
+            op2.graph_launch(rid, 1)
+            log = op2.reduction_log()
         op2.sync()
         dt = time.perf_counter() - t0
+        last = np.sqrt(log.reshape(-1, 2, batch.nmesh)[-1, 1] / af.mesh_cells)
         op2.set_deferred_reductions(False)
     finally:
         op2.op_exit()
@@ -209,15 +236,39 @@ def measure_batch(a, batch, dtype):
             "edges": batch.union.nedge,
             "cells_min": int(min(m.ncell for m in batch.meshes)),
             "cells_max": int(max(m.ncell for m in batch.meshes)),
-            "rms_last_min": float(hist[-1].min()), "rms_last_max": float(hist[-1].max()),
-            "tuned": tuned, "loops": loops}
+            "rms_last_min": float(last.min()), "rms_last_max": float(last.max()),
+            "graph_iters": k, "tuned": tuned, "loops": loops}
+
+
+def batch_accuracy(batch, iters, renumber, graph_iters):
+    """The batch in the FP32 form against FP64 after `iters` iterations
+    (fused graphs, as timed): per-mesh rms-history relative error and
+    max |q - q64| relative to the mesh's max |q|, worst over the meshes."""
+    from oplus2_matlab_amd import airfoil as AF
+    from oplus2_matlab_amd import op2
+
+    out = {}
+    for dtype in ("double", "float"):
+        op2.op_init([], 0)
+        try:
+            out[dtype] = AF.run_airfoil_batch(batch, iters, dtype, graph=True, renumber=renumber,
+                                              graph_iters=max(1, min(graph_iters, iters - 1)))
+        finally:
+            op2.op_exit()
+    (h64, q64), (h32, q32) = out["double"], out["float"]
+    worst = worst_q = 0.0
+    for i in range(batch.nmesh):
+        lo, hi = batch.cell_off[i], batch.cell_off[i + 1]
+        worst = max(worst, float(np.max(np.abs(h32[:, i] - h64[:, i]) / np.abs(h64[:, i]))))
+        worst_q = max(worst_q, float(np.max(np.abs(q32[lo:hi].astype(np.float64) - q64[lo:hi])) /
+                                     np.max(np.abs(q64[lo:hi]))))
+    return {"iterations": iters, "meshes": batch.nmesh,
+            "rms_history_max_rel_err": worst, "max_abs_q_minus_q64_rel": worst_q}
 
 
 def accuracy(mesh, consts, iters, renumber, graph_iters=1):
     """FP32 form against FP64 after `iters` iterations, replayed as graphs of
     `graph_iters` fused iterations (the path the throughput figures time)."""
-    import numpy as np
-
     from oplus2_matlab_amd import airfoil as AF
     from oplus2_matlab_amd import op2
 
@@ -227,23 +278,45 @@ def accuracy(mesh, consts, iters, renumber, graph_iters=1):
         try:
             out[dtype] = AF.run_airfoil(mesh, consts, iters, dtype, graph=True,
                                         renumber=renumber,
-                                        graph_iters=math.gcd(max(graph_iters, 1), iters))
+                                        graph_iters=max(1, min(graph_iters, iters - 1)))
         finally:
             op2.op_exit()
     h64, q64 = out["double"][0], out["double"][1]
     h32, q32 = out["float"][0], out["float"][1]
-    return {"iterations": iters, "graph_iters": math.gcd(max(graph_iters, 1), iters),
+    return {"iterations": iters, "graph_iters": max(1, min(graph_iters, iters - 1)),
             "rms_history_max_rel_err": float(np.max(np.abs(h32 - h64) / np.abs(h64))),
             "rms_final_rel_err": float(abs(h32[-1] - h64[-1]) / abs(h64[-1])),
             "max_abs_q_minus_q64": float(np.max(np.abs(q32.astype(np.float64) - q64)))}
 
 
-def cpu_baseline(mesh, consts, target_s):
-    import numpy as np
+def host_cpus():
+    """The host's CPU model (/proc/cpuinfo, as lscpu reports it), its logical
+    CPU count, the CPUs this process may run on, and the cgroup CPU quota
+    (cpu.max; None when unlimited or unreadable)."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    quota = None
+    try:
+        q, period = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if q != "max":
+            quota = int(q) / int(period)
+    except (OSError, ValueError):
+        pass
+    return {"model": model, "logical_cpus": os.cpu_count(),
+            "affinity_cpus": len(os.sched_getaffinity(0)), "cgroup_cpu_quota": quota}
 
+
+def cpu_baseline(mesh, consts, target_s):
     from oracle import oracle
 
     nthr = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+    host = host_cpus()
     # five iterations size the sample (the first, cold one alone costs about
     # two later ones); loop_seconds excludes plan construction
     _, _, secs = oracle.airfoil_omp(mesh, consts, 5, "f64", nthreads=nthr)
@@ -252,7 +325,7 @@ def cpu_baseline(mesh, consts, target_s):
     _, _, secs = oracle.airfoil_omp(mesh, consts, iters, "f64", nthreads=nthr)
     t = float(np.sum(secs))
     return {"value": mesh.ncell * iters / t, "unit": "cell-updates/s", "cores": nthr,
-            "kind": "port",
+            "kind": "port", "host": host,
             "sample": f"{iters} iterations of the same FP64 airfoil mesh on the reference "
                       f"OpenMP schedule (oracle restatement, bit-exact to the reference, "
                       f"adaptive part size), {nthr} threads, {t:.1f} s",
@@ -357,6 +430,9 @@ def main():
             rb["fp32_form"] = {"value": rb32["value"], "ms_per_step": rb32["ms_per_step"],
                                "speedup_vs_fp64": rb32["value"] / rb["value"],
                                "tuned": rb32["tuned"], "loops": rb32["loops"]}
+            if a.accuracy_iters > 0:
+                rb["fp32_form"]["accuracy"] = batch_accuracy(
+                    batch, min(a.accuracy_iters, 100), a.renumber == "on", a.graph_iters)
             if "fp32_form" in result:
                 rb["fp32_form"]["ratio_vs_single_mesh"] = rb32["value"] / result["fp32_form"]["value"]
         result["batched"] = rb

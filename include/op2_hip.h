@@ -59,8 +59,17 @@ void op_diagnostic_output(void);
  * device time of the loop's kernels (HIP events) */
 void op_timing_output(void);
 
-/* common/op_lib_core.c:627-1089: returns the runtime's plan for the loop as
- * an op_plan view (host arrays of the block layout and colouring) */
+/* common/op_lib_core.c:627-1089: the execution plan of a loop in the
+ * reference's layout, built on the host and kept until op_exit: blocks of
+ * part_size consecutive elements (0: the reference's size for 48 KB of
+ * staged data per 64 elements); per indirect dataset inds[m] (0..ninds-1,
+ * -1 for direct and global arguments) the ascending distinct targets of
+ * each block (ind_maps, ind_offs, ind_sizes, nindirect) and each indirect
+ * argument's 16-bit local index per element (loc_maps); thread colours over
+ * the OP_INC arguments (thrcol, nthrcol); block colours (ncolors, ncolblk,
+ * blkmap); nshared; the transfer / transfer2 byte model.  The runtime's
+ * own kernels run on plans of their own layout (op2_plan_summary_get).
+ * Returns NULL (with the error recorded) on invalid input. */
 op_plan *op_plan_core(char const *name, op_set set, int part_size, int nargs,
                       op_arg *args, int ninds, int *inds);
 
@@ -81,6 +90,9 @@ extern op_set *OP_set_list;
 extern op_map *OP_map_list;
 extern op_dat *OP_dat_list;
 extern op_kernel *OP_kernels;
+/* common/op_lib.h:85: the plans op_plan_core has built (OP_plan_index of
+ * them), in the reference's layout */
+extern op_plan *OP_plans;
 
 /* ---------------------------------------------------------------------------
  * C-ABI forms of the reference's C++-linkage entry points
@@ -210,6 +222,8 @@ int op2_renumber_export(op_set set, int *new_of_old);
  * each loop by name; op_par_loop / op2_par_loop then run a registered loop
  * whenever no built-in kernel matches the loop's name and arguments. */
 typedef int (*op2_loop_fn)(char const *name, op_set set, int nargs, op_arg *args);
+/* A registered loop runs the user's kernel even where a built-in kernel has
+ * the same name and argument signature.  fn NULL removes the registration. */
 int op2_register_loop(char const *name, op2_loop_fn fn);
 
 /* Device copy of a constant: called with the constant's bytes and the
@@ -332,6 +346,7 @@ typedef struct {
   double mean_thread_colours;
   long long verify_errors;
   int max_incs;
+  int cached_plans; /* execution plans the runtime holds after this call */
 } op2_plan_summary;
 int op2_plan_summary_get(char const *name, op_set set, int nargs, op_arg *args, int strategy,
                          int part_size, op2_plan_summary *out);

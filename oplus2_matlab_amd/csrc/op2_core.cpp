@@ -20,6 +20,7 @@ op_set *OP_set_list = nullptr;
 op_map *OP_map_list = nullptr;
 op_dat *OP_dat_list = nullptr;
 op_kernel *OP_kernels = nullptr;
+op_plan *OP_plans = nullptr;
 }
 
 namespace op2 {
@@ -152,17 +153,24 @@ LoopConfig config_for(const std::string &loop) {
 
 // ---- timing -----------------------------------------------------------------
 
-LoopTiming &timing_for(const std::string &name, double bytes_per_call) {
+LoopTiming &timing_for(const std::string &name, double bytes_per_call, bool indirect) {
   Runtime &r = rt();
   for (auto &t : r.timings)
     if (t.name == name) {
       t.bytes_per_call = bytes_per_call;
+      t.indirect = indirect;
       return t;
     }
   r.timings.emplace_back();
-  r.timings.back().name = name;
-  r.timings.back().bytes_per_call = bytes_per_call;
-  return r.timings.back();
+  LoopTiming &t = r.timings.back();
+  t.name = name;
+  t.bytes_per_call = bytes_per_call;
+  t.indirect = indirect;
+  // kernel numbers in first-call order, as op2.m numbers a program's loops
+  t.kernel = (int)r.timings.size() - 1;
+  op_timing_realloc(t.kernel);
+  OP_kernels[t.kernel].name = t.name.c_str();
+  return t;
 }
 
 static hipEvent_t take_event() {
@@ -179,6 +187,7 @@ static hipEvent_t take_event() {
 
 void timing_begin(LoopTiming &t, hipEvent_t *start) {
   (void)t;
+  rt().last_plan = nullptr;
   *start = nullptr;
   if (!rt().timing_enabled || rt().capturing) return;
   *start = take_event();
@@ -187,6 +196,17 @@ void timing_begin(LoopTiming &t, hipEvent_t *start) {
 
 void timing_end(LoopTiming &t, hipEvent_t start) {
   t.count++;
+  // the reference's record (e.g. airfoil/res_calc_kernel.cpp:241-248):
+  // transfer = bytes the loop uses (here each touched dat once per
+  // direction and each map once); transfer2, for loops through maps only,
+  // adds the bytes of the plan arrays the kernels read
+  op_kernel &k = OP_kernels[t.kernel];
+  k.count = t.count;
+  k.transfer += (float)t.bytes_per_call;
+  if (t.indirect) {
+    const Plan *p = rt().last_plan;
+    k.transfer2 += (float)(t.bytes_per_call + (p ? (double)p->device_bytes : 0.0));
+  }
   if (!start) return;
   hipEvent_t stop = take_event();
   OP2_HIP_CHECK(hipEventRecord(stop, rt().stream));
@@ -207,6 +227,7 @@ void timing_drain() {
       r.event_pool.push_back(p.stop);
     }
     t.pending.clear();
+    if (t.kernel >= 0 && t.kernel < OP_kern_max) OP_kernels[t.kernel].time = (float)t.seconds;
   }
 }
 
@@ -228,6 +249,7 @@ static void clear_all() {
   r.deferred = false;
   for (auto &p : r.plans) p->release();
   r.plans.clear();
+  release_ref_plans();
   release_exec_caches();
   for (int i = 0; i < OP_dat_index; i++) {
     dev_free(OP_dat_list[i]->data_d);
@@ -529,17 +551,25 @@ void op_diagnostic_output(void) {
   printf("\n");
 }
 
+// The reference's table (common/op_lib_core.c:381-403): per loop its call
+// count, seconds (device time of its kernels, HIP events) and GB/s of
+// transfer and, for loops through maps, of transfer2.
 void op_timing_output(void) {
   API_BEGIN
   timing_drain();
-  Runtime &r = rt();
-  if (r.timings.empty()) return;
-  printf("\n  count     time     GB/s     kernel name ");
-  printf("\n -------------------------------------------- \n");
-  for (auto &t : r.timings) {
-    if (!t.count) continue;
-    double gbs = t.seconds > 0 ? t.bytes_per_call * t.count / (1e9 * t.seconds) : 0.0;
-    printf(" %6d  %8.4f %8.2f     %s \n", t.count, t.seconds, gbs, t.name.c_str());
+  if (OP_kern_max <= 0) return;
+  printf("\n  count     time     GB/s     GB/s   kernel name ");
+  printf("\n ----------------------------------------------- \n");
+  for (int n = 0; n < OP_kern_max; n++) {
+    const op_kernel &k = OP_kernels[n];
+    if (k.count <= 0) continue;
+    const float t = k.time > 0 ? k.time : 1e-30f;
+    if (k.transfer2 == 0.0f)
+      printf(" %6d  %8.4f %8.4f            %s \n", k.count, k.time, k.transfer / (1e9f * t),
+             k.name);
+    else
+      printf(" %6d  %8.4f %8.4f %8.4f   %s \n", k.count, k.time, k.transfer / (1e9f * t),
+             k.transfer2 / (1e9f * t), k.name);
   }
   API_END()
 }
@@ -629,6 +659,10 @@ void op2_timing_reset(void) {
   for (auto &t : rt().timings) {
     t.count = 0;
     t.seconds = 0;
+  }
+  for (int n = 0; n < OP_kern_max; n++) {
+    OP_kernels[n].count = 0;
+    OP_kernels[n].time = OP_kernels[n].transfer = OP_kernels[n].transfer2 = 0.0f;
   }
   API_END()
 }
@@ -773,6 +807,7 @@ int op2_autotune_online(char const *name, int reps) {
   API_BEGIN
   if (!name || reps <= 0) fail("op2_autotune_online: invalid arguments");
   if (rt().capturing) fail("op2_autotune_online: not during graph capture");
+  refuse_with_live_graphs("op2_autotune_online");
   OnlineTune t;
   t.reps = reps;
   rt().online[name] = t;
@@ -796,8 +831,11 @@ int op2_autotune_online_result(char const *name, int *best_strategy, int *best_p
 
 int op2_register_loop(char const *name, op2_loop_fn fn) {
   API_BEGIN
-  if (!name || !fn) fail("op2_register_loop: invalid arguments");
-  rt().loops[name] = fn;
+  if (!name) fail("op2_register_loop: invalid arguments");
+  if (fn)
+    rt().loops[name] = fn;
+  else
+    rt().loops.erase(name);
   return 0;
   API_END(-1)
 }

@@ -1066,6 +1066,13 @@ int real_bytes(int nargs, const op_arg *args) {
   return rb < 0 ? 0 : rb;
 }
 
+// true if an argument reaches its dat through a map
+bool through_map(int nargs, const op_arg *args) {
+  for (int i = 0; i < nargs; i++)
+    if (args[i].argtype == OP_ARG_DAT && args[i].map) return true;
+  return false;
+}
+
 // Algorithmic bytes of one call: each touched dat once per direction (read
 // for READ/RW/INC, write for WRITE/RW/INC), each map array once; an
 // indirect dat counts min(target set size, slots x loop set size) elements.
@@ -1506,6 +1513,7 @@ void run_bres_calc(const char *name, op_set set, int nargs, op_arg *a) {
     const PlanGroup &cells = p->groups[p->owner_group];
     const size_t lds = owner_lds<T>(p, 2, 4, 4);
     allow_lds(k_bres_calc_owner<T>, lds);
+    if (p->nblocks == 0) return;
     hipLaunchKernelGGL(k_bres_calc_owner<T>, dim3(p->nblocks), dim3(NT), lds, rt().stream,
                        p->d_blk_off, p->nblocks, p->d_perm, nodes.d_off, nodes.d_list,
                        cells.d_off, cells.d_list, (const ushort4 *)p->d_loc, p->d_owned_off,
@@ -1729,6 +1737,7 @@ void run_jac_res(const char *name, op_set set, int nargs, op_arg *a) {
     const PlanGroup &nodes = p->groups[0];
     const size_t lds = sizeof(T) * ((size_t)nodes.max_count + p->max_positions);
     allow_lds(k_jac_res_owner<T>, lds);
+    if (p->nblocks == 0) return;
     hipLaunchKernelGGL(k_jac_res_owner<T>, dim3(p->nblocks), dim3(NT), lds, rt().stream,
                        p->d_blk_off, p->nblocks, p->d_perm, nodes.d_off, nodes.d_list,
                        (const ushort2 *)p->d_loc, p->d_owned_off, p->d_owned, p->d_csr_off,
@@ -2080,8 +2089,11 @@ bool overwrites_held_res(const char *name, Loop l, op_set set, int nargs, op_arg
 // increments.
 bool takes_held_bres(Loop l, int rb, op_set set, const op_arg *a) {
   const Held &h = held();
+  // the fused kernel applies the boundary flux with update's adt (a[3]), so
+  // the bres_calc must read that same dat
   return h.active && h.l == Loop::kBresCalc && (l == Loop::kUpdate || l == Loop::kUpdateBatch) &&
-         h.rb == rb && h.args[2].map->to == set && h.args[4].dat == a[2].dat;
+         h.rb == rb && h.args[2].map->to == set && h.args[4].dat == a[2].dat &&
+         h.args[3].dat == a[3].dat;
 }
 
 // A save_soln call (l, rb, set, a) that copies the q a held update writes.
@@ -2140,16 +2152,19 @@ int run_loop(const char *name, op_set set, int nargs, op_arg *args) {
            m, name);
     if (args[m].map) ensure_map_device(args[m].map);
   }
-  const Loop l = identify(name, set, nargs, args);
+  // a loop registered by a translation unit of the generic path runs the
+  // user's own kernel, even where a built-in kernel has the same name and
+  // argument signature (the user's body may differ from the built-in one)
+  auto reg = rt().loops.find(name);
+  const Loop l = reg != rt().loops.end() ? Loop::kUnknown : identify(name, set, nargs, args);
   if (l == Loop::kUnknown) {
     release_held(nullptr);
     flush_fold();
-    // a loop registered by a translation unit of the generic path
-    auto it = rt().loops.find(name);
+    auto it = reg;
     if (it == rt().loops.end())
       fail("op_par_loop error -- no device kernel for loop \"%s\" with this argument "
            "signature, and no registered generic loop of that name", name);
-    LoopTiming &t = timing_for(name, loop_bytes(set, nargs, args));
+    LoopTiming &t = timing_for(name, loop_bytes(set, nargs, args), through_map(nargs, args));
     hipEvent_t start;
     timing_begin(t, &start);
     const int rc = it->second(name, set, nargs, args);
@@ -2206,7 +2221,7 @@ int run_loop(const char *name, op_set set, int nargs, op_arg *args) {
   }
 
   if (l != Loop::kResCalc) flush_fold();  // res_calc may carry it
-  LoopTiming &t = timing_for(name, loop_bytes(set, nargs, args));
+  LoopTiming &t = timing_for(name, loop_bytes(set, nargs, args), through_map(nargs, args));
   hipEvent_t start;
   timing_begin(t, &start);
   if (l == Loop::kUpdate) {
@@ -2262,7 +2277,8 @@ int run_loop(const char *name, op_set set, int nargs, op_arg *args) {
 int autotune(const char *name, op_set set, int nargs, op_arg *args, int reps, int *best_strategy,
              int *best_part, double *best_us) {
   ensure_device();
-  const Loop l = identify(name, set, nargs, args);
+  refuse_with_live_graphs("op2_autotune_loop");
+  const Loop l = rt().loops.count(name) ? Loop::kUnknown : identify(name, set, nargs, args);
   if (l != Loop::kResCalc && l != Loop::kBresCalc && l != Loop::kJacRes)
     fail("op2: loop \"%s\" cannot be autotuned (only indirect loops without global results)",
          name);

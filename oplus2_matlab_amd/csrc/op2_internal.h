@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <deque>
 #include <map>
 #include <memory>
 #include <stdexcept>
@@ -115,19 +116,44 @@ struct Plan {
   int *d_owned_off = nullptr, *d_owned = nullptr, *d_csr_off = nullptr;
   uint16_t *d_csr = nullptr, *d_ell = nullptr;
 
-  // the op_plan view returned by op_plan_core / op_plan_get
-  op_plan view{};
-  std::vector<int> view_offset, view_nelems, view_ncolblk, view_nthrcol;
-  int count = 0;
+  int count = 0;  // calls that used this plan
 
   bool uploaded = false;
+  size_t device_bytes = 0;  // bytes of the device arrays (read once per call)
   void upload();  // no-op once uploaded
-  void release();
+  // frees the device arrays, or moves their addresses to *keep when given
+  void release(std::vector<void *> *keep = nullptr);
 };
 
-// Per-loop timing record (the op_kernel table of op_lib_core.c:381-427).
+// An execution plan in the reference's own layout (common/op_lib_core.h:
+// 122-150), built for callers of op_plan_core / op_plan_get: blocks of
+// part_size consecutive elements; per indirect dataset (the caller's inds)
+// the block's distinct targets in ascending order (ind_maps, ind_offs,
+// ind_sizes) and every indirect argument's 16-bit local index per element
+// (loc_maps); thread colours over the OP_INC arguments inside each block;
+// block colours; shared-memory bytes and the transfer byte model.  The
+// runtime's kernels do not use it (they run on Plan).
+struct RefPlan {
+  std::string name;
+  op_plan view{};
+  std::vector<op_map> maps;
+  std::vector<op_dat> dats;
+  std::vector<int> idxs, inds;
+  std::vector<op_access> accs;
+  std::vector<int> nthrcol, thrcol, offset, ind_offs, ind_sizes, nindirect, nelems, ncolblk,
+      blkmap;
+  std::vector<std::vector<int>> ind_maps;
+  std::vector<int *> ind_map_ptrs;
+  std::vector<std::vector<short>> loc_maps;
+  std::vector<short *> loc_map_ptrs;
+};
+
+// Per-loop timing record (the op_kernel table of op_lib_core.c:381-427);
+// OP_kernels[kernel] mirrors it (kernel numbers in first-call order).
 struct LoopTiming {
   std::string name;
+  int kernel = -1;
+  bool indirect = false;  // has an argument through a map
   int count = 0;
   double seconds = 0;
   double bytes_per_call = 0;
@@ -160,7 +186,9 @@ struct Runtime {
   std::map<std::string, LoopConfig> loop_configs;
   LoopConfig default_config;
   std::vector<std::unique_ptr<Plan>> plans;
-  std::vector<LoopTiming> timings;
+  std::vector<std::unique_ptr<RefPlan>> ref_plans;  // op_plan_core views (OP_plans)
+  std::deque<LoopTiming> timings;  // deque: OP_kernels[k].name points into it
+  const Plan *last_plan = nullptr;  // the last plan get_plan returned
   std::vector<hipEvent_t> event_pool;
   // reduction scratch: per-block partials + ticket counter + result slots
   void *d_reduct = nullptr;
@@ -243,11 +271,19 @@ struct PlanRequest {
 };
 Plan *get_plan(const PlanRequest &req);
 long long verify_plan(const Plan &p);
-// releases a plan's device arrays and forgets it
+// frees the op_plan_core views and the OP_plans array (op_exit)
+void release_ref_plans();
+// releases a plan's device arrays and forgets it (while a live graph may
+// hold their addresses the arrays are kept until op_exit instead)
 void drop_plan(Plan *p);
+// true while an instantiated graph exists (op2_graph_end, not destroyed)
+bool live_graphs();
+// fails with "<what>: ..." while live_graphs(): the call would free or
+// move device arrays that a captured graph reads
+void refuse_with_live_graphs(const char *what);
 
 // timing (op2_core.cpp)
-LoopTiming &timing_for(const std::string &name, double bytes_per_call);
+LoopTiming &timing_for(const std::string &name, double bytes_per_call, bool indirect);
 void timing_begin(LoopTiming &t, hipEvent_t *start);
 void timing_end(LoopTiming &t, hipEvent_t start);
 void timing_drain();

@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <cstring>
 #include <numeric>
+#include <tuple>
 
 #include "op2_internal.h"
 
@@ -41,7 +42,9 @@ static inline int target(op_map m, int idx, int e) {
 void Plan::upload() {
   if (uploaded) return;
   uploaded = true;
-  auto up = [](const void *src, size_t bytes) -> void * {
+  device_bytes = 0;
+  auto up = [this](const void *src, size_t bytes) -> void * {
+    device_bytes += bytes;
     void *d = dev_alloc(bytes);
     if (src) h2d(d, src, bytes);  // (an empty vector has no data: nothing to copy)
     return d;
@@ -69,18 +72,26 @@ void Plan::upload() {
   }
 }
 
-void Plan::release() {
+void Plan::release(std::vector<void *> *keep) {
+  auto drop = [keep](void *p) {
+    if (!p) return;
+    if (keep)
+      keep->push_back(p);
+    else
+      dev_free(p);
+  };
   for (void *p : {(void *)d_blk_off, (void *)d_perm, (void *)d_col_off, (void *)d_col_base,
                   (void *)d_blk_ncol, (void *)d_blkmap, (void *)d_loc, (void *)d_owned_off,
                   (void *)d_owned, (void *)d_csr_off, (void *)d_csr, (void *)d_ell,
                   (void *)d_ids})
-    dev_free(p);
+    drop(p);
   d_ids = nullptr;
   d_owned_off = d_owned = d_csr_off = nullptr;
   d_csr = d_ell = nullptr;
   for (auto &g : groups) {
-    dev_free(g.d_off);
-    dev_free(g.d_list);
+    drop(g.d_off);
+    drop(g.d_list);
+    g.d_off = g.d_list = nullptr;
   }
   d_blk_off = d_perm = d_col_off = d_col_base = d_blk_ncol = d_blkmap = nullptr;
   d_loc = nullptr;
@@ -476,27 +487,6 @@ static Plan *build(const PlanRequest &req) {
     fail("op2: plan verification failed for loop %s (%lld violations)", req.loop.c_str(),
          P.verify_errors);
 
-  // reference-shaped view (common/op_lib_core.h:122-150)
-  P.view_offset.assign(P.blk_off.begin(), P.blk_off.end() - 1);
-  P.view_nelems.resize(P.nblocks);
-  for (int b = 0; b < P.nblocks; b++) P.view_nelems[b] = P.blk_off[b + 1] - P.blk_off[b];
-  P.view_ncolblk.resize(P.ncolors);
-  for (int c = 0; c < P.ncolors; c++) P.view_ncolblk[c] = P.colblk_off[c + 1] - P.colblk_off[c];
-  P.view_nthrcol = P.blk_ncol.empty() ? std::vector<int>(P.nblocks, 1) : P.blk_ncol;
-  op_plan &v = P.view;
-  memset(&v, 0, sizeof v);
-  v.set = P.set;
-  v.nargs = req.nargs;
-  v.ninds = ng;
-  v.part_size = req.part_size;
-  v.nblocks = P.nblocks;
-  v.ncolors = P.ncolors;
-  v.offset = P.view_offset.data();
-  v.nelems = P.view_nelems.data();
-  v.ncolblk = P.view_ncolblk.data();
-  v.blkmap = P.blkmap.data();
-  v.nthrcol = P.view_nthrcol.data();
-  v.thrcol = P.thrcol.empty() ? nullptr : P.thrcol.data();
   return p.release();
 }
 
@@ -674,14 +664,26 @@ long long verify_plan(const Plan &P) {
   return err;
 }
 
+bool live_graphs() {
+  for (auto g : rt().graphs)
+    if (g) return true;
+  return false;
+}
+
+void refuse_with_live_graphs(const char *what) {
+  if (live_graphs())
+    fail("%s: captured graphs exist and may use the plans it would replace; destroy them "
+         "(op2_graph_destroy) first", what);
+}
+
 void drop_plan(Plan *p) {
   Runtime &r = rt();
   for (size_t i = 0; i < r.plans.size(); i++)
     if (r.plans[i].get() == p) {
       if (r.device_ready) (void)hipStreamSynchronize(r.stream);
-      p->release();
+      p->release(live_graphs() ? &r.retired : nullptr);
+      if (r.last_plan == p) r.last_plan = nullptr;
       r.plans.erase(r.plans.begin() + i);
-      OP_plan_index = (int)r.plans.size();
       return;
     }
 }
@@ -690,8 +692,8 @@ Plan *get_plan(const PlanRequest &req) {
   Runtime &r = rt();
   for (auto &p : r.plans)
     if (same_key(*p, req)) {
+      r.last_plan = p.get();
       p->count++;
-      p->view.count = p->count;
       if (OP_diags > 3) printf(" old execution plan for kernel %s\n", req.loop.c_str());
       return p.get();
     }
@@ -702,10 +704,8 @@ Plan *get_plan(const PlanRequest &req) {
   op_timers(&cpu, &t1);
   if (OP_diags > 1) printf(" plan built and verified in %.3f s\n", t1 - t0);
   p->count = 1;
-  p->view.count = 1;
   r.plans.emplace_back(p);
-  p->view.name = p->loop.c_str();
-  OP_plan_index = (int)r.plans.size();
+  r.last_plan = p;
   if (OP_diags > 1) {
     printf(" number of blocks       = %d \n", p->nblocks);
     printf(" number of block colors = %d \n", p->ncolors);
@@ -713,6 +713,380 @@ Plan *get_plan(const PlanRequest &req) {
     printf(" average thread colors  = %.2f \n", p->mean_thread_colours);
   }
   return p;
+}
+
+// ---- plans in the reference's layout (op_plan_core / op_plan_get) ----------
+
+// the reference's key (common/op_lib_core.c:632-645) plus the dataset
+// numbering inds, which shapes the plan too
+static bool same_ref_key(const RefPlan &p, const char *name, op_set set, int part_size,
+                         int nargs, const op_arg *args, int ninds, const int *inds) {
+  if (p.name != name || p.view.set != set || p.view.part_size != part_size ||
+      p.view.nargs != nargs || p.view.ninds != ninds)
+    return false;
+  for (int m = 0; m < nargs; m++)
+    if (p.dats[m] != args[m].dat || p.maps[m] != args[m].map || p.idxs[m] != args[m].idx ||
+        p.accs[m] != args[m].acc || p.inds[m] != (ninds > 0 ? inds[m] : -1))
+      return false;
+  return true;
+}
+
+// OP_plans holds a copy of every view (its arrays point into the RefPlans),
+// refreshed whenever a view is added or its call count changes.
+static void publish_ref_plans() {
+  Runtime &r = rt();
+  const int n = (int)r.ref_plans.size();
+  if (n > OP_plan_max) {
+    op_plan *np = (op_plan *)realloc(OP_plans, sizeof(op_plan) * (size_t)(n + 10));
+    if (!np) fail(" op_plan error -- error reallocating memory for OP_plans");
+    OP_plans = np;
+    OP_plan_max = n + 10;
+  }
+  for (int i = 0; i < n; i++) OP_plans[i] = r.ref_plans[i]->view;
+  OP_plan_index = n;
+}
+
+void release_ref_plans() {
+  rt().ref_plans.clear();
+  free(OP_plans);
+  OP_plans = nullptr;
+  OP_plan_index = OP_plan_max = 0;
+}
+
+// Exact check of a view: every local index maps back to the caller's map
+// through ind_maps; per block and dataset the targets are distinct and
+// ascending; blocks cover the set in order; blkmap is a permutation grouped
+// by colour; no two elements of one thread colour in a block, and no two
+// blocks of one colour, share an OP_INC target.
+static long long check_ref_plan(const RefPlan &P) {
+  const op_plan &v = P.view;
+  const int n = v.set->size, nb = v.nblocks, ni = v.ninds;
+  long long err = 0;
+  int pos = 0;
+  for (int b = 0; b < nb; b++) {
+    err += P.offset[b] != pos;
+    pos += P.nelems[b];
+    for (int i = 0; i < ni; i++) {
+      const int o = P.ind_offs[i + b * ni], sz = P.ind_sizes[i + b * ni];
+      for (int k = o + 1; k < o + sz; k++) err += P.ind_maps[i][k] <= P.ind_maps[i][k - 1];
+    }
+  }
+  err += pos != n;
+  for (int m = 0; m < v.nargs; m++) {
+    if (P.inds[m] < 0) continue;
+    const int i = P.inds[m];
+    const int *mp = map_host(P.maps[m]);
+    for (int b = 0; b < nb; b++)
+      for (int e = P.offset[b]; e < P.offset[b] + P.nelems[b]; e++) {
+        const int l = P.loc_maps[m][e];
+        if (l < 0 || l >= P.ind_sizes[i + b * ni]) {
+          err++;
+          continue;
+        }
+        err += P.ind_maps[i][P.ind_offs[i + b * ni] + l] != mp[(size_t)e * P.maps[m]->dim + P.idxs[m]];
+      }
+  }
+  std::vector<int> seen(nb, 0);
+  int k = 0;
+  for (int c = 0; c < v.ncolors; c++)
+    for (int j = 0; j < P.ncolblk[c]; j++, k++) {
+      const int b = P.blkmap[k];
+      if (b < 0 || b >= nb || seen[b]++) err++;
+    }
+  err += k != nb;
+  // colour clashes: (dataset, target, colour) claimed by two owners
+  std::vector<std::tuple<int, int, int, int>> t;
+  auto clashes = [&t]() {
+    std::sort(t.begin(), t.end());
+    long long e = 0;
+    for (size_t a = 1; a < t.size(); a++)
+      e += std::get<0>(t[a]) == std::get<0>(t[a - 1]) && std::get<1>(t[a]) == std::get<1>(t[a - 1]) &&
+           std::get<2>(t[a]) == std::get<2>(t[a - 1]) && std::get<3>(t[a]) != std::get<3>(t[a - 1]);
+    t.clear();
+    return e;
+  };
+  std::vector<int> blkcol(nb, 0);
+  k = 0;
+  for (int c = 0; c < v.ncolors; c++)
+    for (int j = 0; j < P.ncolblk[c]; j++, k++)
+      if (k < nb && P.blkmap[k] >= 0 && P.blkmap[k] < nb) blkcol[P.blkmap[k]] = c;
+  for (int b = 0; b < nb; b++) {
+    for (int e = P.offset[b]; e < P.offset[b] + P.nelems[b]; e++) {
+      err += P.thrcol[e] < 0 || P.thrcol[e] >= P.nthrcol[b];
+      for (int m = 0; m < v.nargs; m++)
+        if (P.inds[m] >= 0 && P.accs[m] == OP_INC)
+          t.emplace_back(P.inds[m], map_host(P.maps[m])[(size_t)e * P.maps[m]->dim + P.idxs[m]],
+                         P.thrcol[e], e);
+    }
+    err += clashes();
+  }
+  for (int b = 0; b < nb; b++)
+    for (int e = P.offset[b]; e < P.offset[b] + P.nelems[b]; e++)
+      for (int m = 0; m < v.nargs; m++)
+        if (P.inds[m] >= 0 && P.accs[m] == OP_INC)
+          t.emplace_back(P.inds[m], map_host(P.maps[m])[(size_t)e * P.maps[m]->dim + P.idxs[m]],
+                         blkcol[b], b);
+  err += clashes();
+  return err;
+}
+
+// Builds (or finds) the view of op_plan_core (common/op_lib_core.c:627-1089
+// defines the layout and its fields): same inputs, same block partition
+// (part_size, or with 0 the size that fits the staged data of 64 elements
+// per 48 KB), same per-dataset ascending target lists, same greedy first-fit
+// colouring in element order, 32 colours per pass (thread colours on the
+// OP_INC arguments inside a block, then block colours), blkmap in block
+// order within each colour, and the same nshared and float byte model.
+op_plan *ref_plan(const char *name, op_set set, int part_size, int nargs, op_arg *args, int ninds,
+                  int *inds) {
+  Runtime &r = rt();
+  if (!name || !set || nargs <= 0 || !args || ninds < 0 || (ninds > 0 && !inds))
+    fail("op_plan_core error -- invalid arguments");
+  for (auto &p : r.ref_plans)
+    if (same_ref_key(*p, name, set, part_size, nargs, args, ninds, inds)) {
+      if (OP_diags > 3) printf(" old execution plan for kernel %s\n", name);
+      p->view.count++;
+      publish_ref_plans();
+      return &p->view;
+    }
+  if (OP_diags > 1) printf(" new execution plan #%zu for kernel %s\n", r.ref_plans.size(), name);
+
+  // each dataset is one dat through one map; every dataset has an argument
+  std::vector<int> first(ninds, -1);
+  int maxbytes = 0;
+  for (int m = 0; m < nargs; m++) {
+    const int i = ninds > 0 ? inds[m] : -1;
+    if (i < -1 || i >= ninds) fail("op_plan_core error -- inds[%d] = %d out of range", m, i);
+    if (i < 0) continue;
+    if (args[m].argtype != OP_ARG_DAT || !args[m].map || !args[m].dat || args[m].idx < 0 ||
+        args[m].idx >= args[m].map->dim || args[m].map->from != set)
+      fail("op_plan_core error -- argument %d of %s is not an indirect dat argument", m, name);
+    if (first[i] < 0)
+      first[i] = m;
+    else if (args[first[i]].dat != args[m].dat || args[first[i]].map != args[m].map)
+      fail("op_plan_core error -- indirect dataset %d of %s names two dats or maps", i, name);
+    maxbytes += args[m].dat->size;
+  }
+  for (int i = 0; i < ninds; i++)
+    if (first[i] < 0) fail("op_plan_core error -- indirect dataset %d of %s has no argument", i, name);
+  int bsize = part_size;
+  if (bsize == 0) {
+    if (maxbytes == 0) fail("op_plan_core error -- %s has no indirect arguments", name);
+    bsize = (48 * 1024 / (64 * maxbytes)) * 64;
+  }
+  if (bsize <= 0) fail("op_plan_core error -- block size %d for %s", bsize, name);
+  const int n = set->size;
+  const int nb = (n - 1) / bsize + 1;
+
+  auto P = std::make_unique<RefPlan>();
+  P->name = name;
+  for (int m = 0; m < nargs; m++) {
+    P->dats.push_back(args[m].dat);
+    P->maps.push_back(args[m].map);
+    P->idxs.push_back(args[m].idx);
+    P->accs.push_back(args[m].acc);
+    P->inds.push_back(ninds > 0 ? inds[m] : -1);
+  }
+  inds = P->inds.data();
+  auto tgt = [&](int m, int e) {
+    return map_host(args[m].map)[(size_t)e * args[m].map->dim + args[m].idx];
+  };
+  P->offset.resize(nb);
+  P->nelems.resize(nb);
+  P->nthrcol.resize(nb);
+  P->thrcol.assign(n, -1);
+  P->ind_offs.resize((size_t)nb * ninds);
+  P->ind_sizes.resize((size_t)nb * ninds);
+  P->nindirect.assign(ninds, 0);
+  P->ind_maps.resize(ninds);
+  P->loc_maps.resize(nargs);
+  for (int m = 0; m < nargs; m++)
+    if (inds[m] >= 0) P->loc_maps[m].assign(n, 0);
+  // per dataset: a colour mask (or a local index) per target element
+  std::vector<std::vector<uint32_t>> work(ninds);
+  for (int i = 0; i < ninds; i++) work[i].assign(args[first[i]].map->to->size, 0);
+
+  std::vector<int> list;
+  double total_colours = 0;
+  for (int b = 0; b < nb; b++) {
+    const int e0 = b * bsize, bs = std::min(bsize, n - e0);
+    P->offset[b] = e0;
+    P->nelems[b] = bs;
+    for (int i = 0; i < ninds; i++) {
+      list.clear();
+      for (int m = 0; m < nargs; m++)
+        if (inds[m] == i)
+          for (int e = e0; e < e0 + bs; e++) list.push_back(tgt(m, e));
+      std::sort(list.begin(), list.end());
+      list.erase(std::unique(list.begin(), list.end()), list.end());
+      if (list.size() > 32767)
+        fail("op_plan_core error -- block of %s stages %zu elements of dataset %d (16-bit local "
+             "indices); reduce the part size", name, list.size(), i);
+      const int o = P->nindirect[i];
+      for (size_t k = 0; k < list.size(); k++) {
+        P->ind_maps[i].push_back(list[k]);
+        work[i][list[k]] = (uint32_t)k;
+      }
+      P->nindirect[i] += (int)list.size();
+      P->ind_offs[i + (size_t)b * ninds] = o;
+      P->ind_sizes[i + (size_t)b * ninds] = (int)list.size();
+      for (int m = 0; m < nargs; m++)
+        if (inds[m] == i)
+          for (int e = e0; e < e0 + bs; e++) P->loc_maps[m][e] = (short)work[i][tgt(m, e)];
+    }
+    // thread colours
+    int base = 0, ncol = 0;
+    for (bool again = true; again; base += 32) {
+      again = false;
+      for (int m = 0; m < nargs; m++)
+        if (inds[m] >= 0)
+          for (int e = e0; e < e0 + bs; e++) work[inds[m]][tgt(m, e)] = 0;
+      for (int e = e0; e < e0 + bs; e++) {
+        if (P->thrcol[e] >= 0) continue;
+        uint32_t mask = 0;
+        for (int m = 0; m < nargs; m++)
+          if (inds[m] >= 0 && args[m].acc == OP_INC) mask |= work[inds[m]][tgt(m, e)];
+        if (mask == 0xffffffffu) {
+          again = true;
+          continue;
+        }
+        const int c = __builtin_ctz(~mask);
+        P->thrcol[e] = base + c;
+        ncol = std::max(ncol, base + c + 1);
+        for (int m = 0; m < nargs; m++)
+          if (inds[m] >= 0 && args[m].acc == OP_INC) work[inds[m]][tgt(m, e)] |= 1u << c;
+      }
+    }
+    P->nthrcol[b] = ncol;
+    total_colours += ncol;
+  }
+
+  // block colours
+  std::vector<int> blkcol(nb, -1);
+  int ncolors = 0;
+  for (int base = 0, again = 1; again; base += 32) {
+    again = 0;
+    for (int i = 0; i < ninds; i++) std::fill(work[i].begin(), work[i].end(), 0u);
+    for (int b = 0; b < nb; b++) {
+      if (blkcol[b] >= 0) continue;
+      const int e0 = P->offset[b], e1 = e0 + P->nelems[b];
+      uint32_t mask = 0;
+      for (int m = 0; m < nargs; m++)
+        if (inds[m] >= 0 && args[m].acc == OP_INC)
+          for (int e = e0; e < e1; e++) mask |= work[inds[m]][tgt(m, e)];
+      if (mask == 0xffffffffu) {
+        again = 1;
+        continue;
+      }
+      const int c = __builtin_ctz(~mask);
+      blkcol[b] = base + c;
+      ncolors = std::max(ncolors, base + c + 1);
+      for (int m = 0; m < nargs; m++)
+        if (inds[m] >= 0 && args[m].acc == OP_INC)
+          for (int e = e0; e < e1; e++) work[inds[m]][tgt(m, e)] |= 1u << c;
+    }
+  }
+  P->ncolblk.assign(std::max(ncolors, 1), 0);
+  for (int b = 0; b < nb; b++) P->ncolblk[blkcol[b]]++;
+  {
+    std::vector<int> start(ncolors + 1, 0);
+    for (int c = 0; c < ncolors; c++) start[c + 1] = start[c] + P->ncolblk[c];
+    P->blkmap.resize(nb);
+    for (int b = 0; b < nb; b++) P->blkmap[start[blkcol[b]]++] = b;
+  }
+
+  // shared memory per block and the transfer byte model, accumulated in
+  // float in the order of the reference's loops
+  int nshared = 0;
+  float transfer = 0, transfer2 = 0;
+  const int cl = OP_cache_line_size > 0 ? OP_cache_line_size : 128;
+  for (int b = 0; b < nb; b++) {
+    int bytes = 0;
+    for (int i = 0; i < ninds; i++)
+      bytes += ROUND_UP(P->ind_sizes[i + (size_t)b * ninds] * args[first[i]].dat->size);
+    nshared = std::max(nshared, bytes);
+  }
+  for (int b = 0; b < nb; b++) {
+    for (int m = 0; m < nargs; m++) {
+      if (inds[m] < 0) {
+        const float fac = args[m].acc == OP_READ ? 1.0f : 2.0f;
+        if (args[m].dat) {
+          transfer += fac * P->nelems[b] * args[m].dat->size;
+          transfer2 += fac * P->nelems[b] * args[m].dat->size;
+        }
+      } else {
+        transfer += (float)(P->nelems[b] * sizeof(short));
+        transfer2 += (float)(P->nelems[b] * sizeof(short));
+      }
+    }
+    for (int i = 0; i < ninds; i++) {
+      const op_arg &a = args[first[i]];
+      const long long sz = a.dat->size;
+      float fac = a.acc == OP_READ ? 1.0f : 2.0f;
+      const int o = P->ind_offs[i + (size_t)b * ninds], cnt = P->ind_sizes[i + (size_t)b * ninds];
+      transfer += fac * cnt * a.dat->size;
+      // cache lines the block's targets occupy
+      long long l_old = -1;
+      for (int e = o; e < o + cnt; e++) {
+        const long long t = P->ind_maps[i][e];
+        long long l_new = (t * sz) / cl;
+        if (l_new > l_old) transfer2 += fac * cl;
+        l_old = l_new;
+        l_new = ((t + 1) * sz - 1) / cl;
+        transfer2 += fac * (l_new - l_old) * cl;
+        l_old = l_new;
+      }
+      // the mapping that loads / stores the staged data
+      fac = a.acc == OP_RW ? 2.0f : 1.0f;
+      transfer += fac * cnt * sizeof(int);
+      transfer2 += fac * cnt * sizeof(int);
+    }
+  }
+
+  for (int i = 0; i < ninds; i++) P->ind_map_ptrs.push_back(P->ind_maps[i].data());
+  for (int m = 0; m < nargs; m++)
+    P->loc_map_ptrs.push_back(P->loc_maps[m].empty() ? nullptr : P->loc_maps[m].data());
+  op_plan &v = P->view;
+  v.name = P->name.c_str();
+  v.set = set;
+  v.nargs = nargs;
+  v.ninds = ninds;
+  v.part_size = part_size;
+  v.maps = P->maps.data();
+  v.dats = P->dats.data();
+  v.idxs = P->idxs.data();
+  v.accs = P->accs.data();
+  v.nthrcol = P->nthrcol.data();
+  v.thrcol = P->thrcol.data();
+  v.offset = P->offset.data();
+  v.ind_maps = P->ind_map_ptrs.data();
+  v.ind_offs = P->ind_offs.data();
+  v.ind_sizes = P->ind_sizes.data();
+  v.nindirect = P->nindirect.data();
+  v.loc_maps = P->loc_map_ptrs.data();
+  v.nblocks = nb;
+  v.nelems = P->nelems.data();
+  v.ncolors = ncolors;
+  v.ncolblk = P->ncolblk.data();
+  v.blkmap = P->blkmap.data();
+  v.nshared = nshared;
+  v.transfer = transfer;
+  v.transfer2 = transfer2;
+  v.count = 1;
+  if (OP_diags > 1) {
+    printf(" number of blocks       = %d \n", nb);
+    printf(" number of block colors = %d \n", ncolors);
+    printf(" maximum block size     = %d \n", bsize);
+    printf(" average thread colors  = %.2f \n", total_colours / nb);
+    printf(" shared memory required = %.2f KB \n", nshared / 1024.0f);
+    printf(" data transfer (used)   = %.2f MB \n", transfer / (1024.0f * 1024.0f));
+    printf(" data transfer (total)  = %.2f MB \n\n", transfer2 / (1024.0f * 1024.0f));
+  }
+  const long long errs = check_ref_plan(*P);
+  if (errs) fail("op_plan_core error -- plan for %s fails its check (%lld violations)", name, errs);
+  r.ref_plans.push_back(std::move(P));
+  publish_ref_plans();
+  return &r.ref_plans.back()->view;
 }
 
 }  // namespace op2
@@ -724,12 +1098,7 @@ extern "C" {
 op_plan *op_plan_core(char const *name, op_set set, int part_size, int nargs, op_arg *args,
                       int ninds, int *inds) {
   try {
-    (void)ninds;
-    (void)inds;
-    LoopConfig c = config_for(name);
-    int strategy = c.strategy ? c.strategy : OP2_STRATEGY_COLOURED;
-    PlanRequest req{name, set, nargs, args, strategy, part_size ? part_size : c.part_size};
-    return &get_plan(req)->view;
+    return ref_plan(name, set, part_size, nargs, args, ninds, inds);
   } catch (const op2::Error &) {
     return nullptr;
   }
@@ -760,6 +1129,7 @@ int op2_plan_summary_get(char const *name, op_set set, int nargs, op_arg *args, 
     int staged = 0;
     for (auto &g : p->groups) staged += g.max_count;
     out->max_staged = staged;
+    out->cached_plans = (int)rt().plans.size();
     return 0;
   } catch (const op2::Error &) {
     return -1;

@@ -67,7 +67,23 @@ class PlanSummary(C.Structure):
     _fields_ = [("nblocks", C.c_int), ("ncolors", C.c_int), ("max_block", C.c_int),
                 ("max_staged", C.c_int), ("strategy", C.c_int), ("part_size", C.c_int),
                 ("mean_thread_colours", C.c_double), ("verify_errors", C.c_longlong),
-                ("max_incs", C.c_int)]
+                ("max_incs", C.c_int), ("cached_plans", C.c_int)]
+
+
+class OpPlan(C.Structure):
+    """op_plan of op_lib_core.h (common/op_lib_core.h:122-150)."""
+    _fields_ = [("name", C.c_char_p), ("set", C.c_void_p), ("nargs", C.c_int),
+                ("ninds", C.c_int), ("part_size", C.c_int),
+                ("maps", C.POINTER(C.c_void_p)), ("dats", C.POINTER(C.c_void_p)),
+                ("idxs", C.POINTER(C.c_int)), ("accs", C.POINTER(C.c_int)),
+                ("nthrcol", C.POINTER(C.c_int)), ("thrcol", C.POINTER(C.c_int)),
+                ("offset", C.POINTER(C.c_int)), ("ind_maps", C.POINTER(C.POINTER(C.c_int))),
+                ("ind_offs", C.POINTER(C.c_int)), ("ind_sizes", C.POINTER(C.c_int)),
+                ("nindirect", C.POINTER(C.c_int)), ("loc_maps", C.POINTER(C.POINTER(C.c_short))),
+                ("nblocks", C.c_int), ("nelems", C.POINTER(C.c_int)), ("ncolors", C.c_int),
+                ("ncolblk", C.POINTER(C.c_int)), ("blkmap", C.POINTER(C.c_int)),
+                ("nshared", C.c_int), ("transfer", C.c_float), ("transfer2", C.c_float),
+                ("count", C.c_int)]
 
 
 # every extern "C" entry point of include/op2_hip.h: name -> (restype, argtypes)
@@ -486,6 +502,41 @@ def plan_export(name: str, set_: Set, args, strategy: int = STRATEGY_COLOURED,
     _call("op2_plan_export", _b(name), set_._p, len(args), arr, strategy, part_size,
           perm.ctypes.data, thr.ctypes.data, off.ctypes.data, col.ctypes.data)
     return {"perm": perm, "thrcol": thr, "blk_off": off, "blkcol": col}
+
+
+def op_plan_get(name: str, set_: Set, args, inds, part_size: int = 0) -> dict:
+    """op_plan_core's plan in the reference's layout (common/op_lib.cu:155-184
+    and op_lib_core.c:627-1089), copied out of the op_plan view: header
+    fields, per-block and per-element arrays, ind_maps per indirect dataset
+    and loc_maps per argument (None for direct and global arguments)."""
+    arr = (OpArg * len(args))(*args)
+    ninds = max(inds) + 1 if len(inds) else 0
+    iv = (C.c_int * len(inds))(*inds)
+    ptr = _call("op_plan_core", _b(name), set_._p, part_size, len(args), arr, ninds, iv)
+    if not ptr:
+        raise OP2Error(lib().op2_last_error().decode())
+    v = OpPlan.from_address(ptr)
+    nb, nc, n = v.nblocks, v.ncolors, set_.size
+
+    def arr_of(p, count, dt=np.int32):
+        return np.ctypeslib.as_array(p, shape=(count,)).astype(dt).copy() if count else \
+            np.zeros(0, dt)
+
+    nind = arr_of(v.nindirect, ninds)
+    return {"nblocks": nb, "ncolors": nc, "nshared": v.nshared, "nargs": v.nargs,
+            "ninds": v.ninds, "part_size": v.part_size, "count": v.count,
+            "transfer": np.float32(v.transfer), "transfer2": np.float32(v.transfer2),
+            "nthrcol": arr_of(v.nthrcol, nb), "thrcol": arr_of(v.thrcol, n),
+            "offset": arr_of(v.offset, nb), "nelems": arr_of(v.nelems, nb),
+            "ncolblk": arr_of(v.ncolblk, nc), "blkmap": arr_of(v.blkmap, nb),
+            "ind_offs": arr_of(v.ind_offs, nb * ninds), "ind_sizes": arr_of(v.ind_sizes, nb * ninds),
+            "nindirect": nind,
+            "ind_maps": [arr_of(v.ind_maps[m], int(nind[m])) for m in range(ninds)],
+            "loc_maps": [arr_of(v.loc_maps[a], n, np.int16) if v.loc_maps[a] else None
+                         for a in range(len(args))],
+            "maps": [v.maps[a] for a in range(len(args))],
+            "dats": [v.dats[a] for a in range(len(args))],
+            "idxs": arr_of(v.idxs, len(args)), "accs": arr_of(v.accs, len(args))}
 
 
 # user-kernel names for op_par_loop's first argument (the reference passes

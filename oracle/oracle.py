@@ -128,6 +128,50 @@ def build_plan(set_size: int, args: list, ninds: int, part_size: int = 0) -> dic
         L.orc_plan_free(C.byref(plan))
 
 
+def check_plan(set_size: int, args: list, ninds: int, p: dict) -> int:
+    """Runs the oracle's plan checks (the reference's op_plan_check,
+    common/op_lib_core.c:469-612, plus the exact colouring check) on a plan
+    given as arrays (the keys build_plan returns); returns the error count."""
+    pin = PlanIn()
+    pin.set_size, pin.nargs, pin.ninds, pin.part_size = set_size, len(args), ninds, 0
+    keep = []
+    for i, a in enumerate(args):
+        if a.map is not None:
+            keep.append(a.map)
+            pin.map[i] = a.map.ctypes.data
+        pin.map_dim[i], pin.idx[i], pin.to_size[i] = a.map_dim, a.idx, a.to_size
+        pin.dat_size[i], pin.dat_dim[i], pin.acc[i], pin.inds[i] = (
+            a.dat_size, a.dat_dim, a.acc, a.ind)
+    nb, nc = p["nblocks"], p["ncolors"]
+    c_int = lambda v: np.ascontiguousarray(v, np.int32)
+    arrays = {k: c_int(p[k]) for k in ("nthrcol", "thrcol", "offset", "nelems", "ncolblk",
+                                        "blkmap", "ind_offs", "ind_sizes")}
+    # block colour of each block, from blkmap grouped by ncolblk
+    blkcol = np.repeat(np.arange(nc, dtype=np.int32), arrays["ncolblk"])
+    arrays["blkcol"] = np.zeros(nb, np.int32)
+    arrays["blkcol"][arrays["blkmap"]] = blkcol
+    keep += list(arrays.values())
+    plan = Plan()
+    plan.nblocks, plan.ncolors, plan.nshared = nb, nc, p["nshared"]
+    plan.ninds, plan.nargs, plan.set_size = ninds, len(args), set_size
+    ip = lambda a: a.ctypes.data_as(C.POINTER(C.c_int))
+    for k, v in arrays.items():
+        setattr(plan, k, ip(v))
+    for m in range(ninds):
+        im = c_int(p["ind_maps"][m])
+        keep.append(im)
+        plan.nindirect[m] = im.size
+        plan.ind_maps[m] = ip(im)
+    for a, lm in enumerate(p["loc_maps"]):
+        if lm is not None:
+            lm = np.ascontiguousarray(lm, np.int16)
+            keep.append(lm)
+            plan.loc_maps[a] = lm.ctypes.data_as(C.POINTER(C.c_short))
+    L = lib()
+    L.orc_plan_check.argtypes = [C.POINTER(PlanIn), C.POINTER(Plan)]
+    return int(L.orc_plan_check(C.byref(pin), C.byref(plan)))
+
+
 def airfoil_plan_args(mesh, loop: str, real_bytes: int) -> tuple:
     """(set_size, args, ninds) of the three indirect airfoil loops, with the
     inds of airfoil/<loop>_kernel.cpp."""
@@ -261,6 +305,18 @@ def jacobi_seq(prob, niter: int, alpha: float, beta: float, form: str = "f64"):
     f(C.c_int(prob.nnode), C.c_int(prob.nedge), _p(prob.pp), _p(A), _p(r), _p(u), _p(du),
       ct(alpha), ct(beta), C.c_int(niter), _p(hs), _p(hm))
     return u, du, hs[:niter], hm[:niter]
+
+
+def jac_update(form, r, du, u, alpha: float):
+    """One jac1 update over every node (jac1/update.h, element order):
+    u += du + alpha r, du = 0; returns (u_sum, u_max) accumulated in the
+    form's reduction type from zero and minus infinity."""
+    real, upd = FORMS[form]
+    s = np.zeros(1, dtype=upd)
+    m = np.full(1, -np.inf, dtype=upd)
+    ct = C.c_double if real == np.float64 else C.c_float
+    _fn("jac_update", form)(C.c_int(u.shape[0]), _p(r), _p(du), _p(u), _p(s), _p(m), ct(alpha))
+    return float(s[0]), float(m[0])
 
 
 def jacobi_omp(prob, niter: int, alpha: float, beta: float, form: str = "f64",

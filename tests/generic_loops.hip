@@ -99,3 +99,22 @@ __device__ inline void gen_res_calc(const double *x1, const double *x2, const do
 
 OP2_DEVICE_LOOP(gen_edge_sum)
 OP2_DEVICE_LOOP(gen_res_calc)
+
+// airfoil update's signature (airfoil/update.h) with another body: half the
+// move.  Registered as "update" only while a test asks for it, to show that
+// a registered loop runs instead of the built-in kernel of the same name and
+// argument signature.
+__device__ inline void half_update(const double *qold, double *q, double *res, const double *adt,
+                                   double *rms) {
+  const double adti = 0.5 / *adt;
+  for (int n = 0; n < 4; n++) {
+    const double del = adti * res[n];
+    q[n] = qold[n] - del;
+    res[n] = 0.0;
+    *rms += del * del;
+  }
+}
+
+extern "C" int testloops_register_half_update(int on) {
+  return op2_register_loop("update", on ? &op2_loop::entry<&half_update> : nullptr);
+}

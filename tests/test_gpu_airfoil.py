@@ -88,6 +88,92 @@ def test_single_loops_fp64(runtime, strategy):
     assert not op2.op_fetch_data(b.p_res).any()
 
 
+EPS32 = float(np.finfo(np.float32).eps)
+
+
+def abs_increments(mesh, x, q, adt, c, res0):
+    """Per cell and component: |res0| plus the sum of |f| over the float
+    res_calc and bres_calc increments the cell receives, each increment
+    taken from the f32 oracle on a one-edge slice of the maps.  A float sum
+    of these terms in any order differs from another order by at most
+    (terms - 1) x eps32 x this sum."""
+    tot = np.abs(res0).astype(np.float64)
+    tmp = np.zeros_like(res0)
+    for e in range(mesh.nedge):
+        one = M.AirfoilMesh(x=mesh.x, cell=mesh.cell, edge=mesh.edge[e:e + 1],
+                            ecell=mesh.ecell[e:e + 1], bedge=mesh.bedge[:0],
+                            becell=mesh.becell[:0], bound=mesh.bound[:0])
+        c1, c2 = mesh.ecell[e]
+        tmp[c1] = tmp[c2] = 0
+        O.res_calc("f32", one, x, q, adt, tmp, c)
+        tot[c1] += np.abs(tmp[c1])
+        tot[c2] += np.abs(tmp[c2])
+    for e in range(mesh.nbedge):
+        one = M.AirfoilMesh(x=mesh.x, cell=mesh.cell, edge=mesh.edge[:0], ecell=mesh.ecell[:0],
+                            bedge=mesh.bedge[e:e + 1], becell=mesh.becell[e:e + 1],
+                            bound=mesh.bound[e:e + 1])
+        c1 = mesh.becell[e]
+        tmp[c1] = 0
+        O.bres_calc("f32", one, x, q, adt, tmp, c)
+        tot[c1] += np.abs(tmp[c1])
+    return tot
+
+
+@pytest.mark.parametrize("strategy", STRATS)
+def test_single_loops_fp32_form(runtime, strategy):
+    """Each float kernel once against the precision-form oracle
+    (orc_*_mix): save_soln and adt_calc bit for bit; res_calc and bres_calc
+    (the same float flux per edge, summed in another order) within
+    4 eps32 of each cell's sum of |increments|; update bit for bit on the
+    same res (move and new q in double, rounded once) and its rms to double
+    rounding."""
+    note(f"single loops fp32 form strategy={strategy} mesh 48x24 random")
+    configure(strategy, 128)
+    mesh = M.o_mesh(48, 24, "random", seed=9)
+    c = M.flow_constants().as_array()
+    rng = np.random.Generator(np.random.PCG64(3))
+    q0 = (np.tile(c[6:10], (mesh.ncell, 1)) *
+          (1 + 0.02 * rng.standard_normal((mesh.ncell, 4)))).astype(np.float32)
+    res0 = (1e-3 * rng.standard_normal((mesh.ncell, 4))).astype(np.float32)
+    b = AF.Airfoil(mesh, c, "float", q0=q0, res0=res0)
+
+    ox, oq = mesh.x.astype(np.float32), q0.copy()
+    oqold = np.zeros_like(oq)
+    O.save_soln("mix", oq, oqold)
+    b.save_soln()
+    assert np.array_equal(op2.op_fetch_data(b.p_qold).reshape(-1, 4), oqold)
+
+    oadt = np.zeros(mesh.ncell, np.float32)
+    O.adt_calc("mix", mesh, ox, oq, oadt, c)
+    b.adt_calc()
+    adt = op2.op_fetch_data(b.p_adt)
+    assert adt.dtype == np.float32 and np.array_equal(adt, oadt)
+
+    bound = 4 * EPS32 * abs_increments(mesh, ox, oq, oadt, c, res0)
+    ores = res0.copy()
+    O.res_calc("mix", mesh, ox, oq, oadt, ores, c)
+    b.res_calc()
+    res = op2.op_fetch_data(b.p_res).reshape(-1, 4)
+    err = np.abs(res.astype(np.float64) - ores)
+    note(f"res_calc: max err / bound {np.max(err / np.maximum(bound, 1e-300)):.3f}, "
+         f"bitwise equal cells {np.mean(np.all(res == ores, axis=1)):.3f}")
+    assert np.all(err <= bound)
+
+    O.bres_calc("mix", mesh, ox, oq, oadt, ores, c)
+    b.bres_calc()
+    res = op2.op_fetch_data(b.p_res).reshape(-1, 4)
+    assert np.all(np.abs(res.astype(np.float64) - ores) <= bound)
+
+    # update on the runtime's own res, so that only update's arithmetic is
+    # compared
+    ures = res.copy()
+    rms_o = O.update("mix", oqold, oq, ures, oadt)
+    rms_g = b.update()
+    assert np.array_equal(op2.op_fetch_data(b.p_q).reshape(-1, 4), oq)
+    assert abs(rms_g - rms_o) / rms_o < 1e-13
+    assert not op2.op_fetch_data(b.p_res).any()
+
+
 # ---------------------------------------------------------------------------
 # full runs against the reference fixtures
 

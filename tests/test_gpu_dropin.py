@@ -25,9 +25,10 @@ DROPIN = os.path.join(ROOT, "oplus2_matlab_amd", "lib", "dropin")
 
 
 def need(name):
+    """The built program; its absence is a failure (build() makes them from
+    the reference sources, and the tree that reaches the GPU carries them)."""
     path = os.path.join(DROPIN, name)
-    if not os.path.exists(path):
-        pytest.skip(f"{path} not built (needs the reference sources at build time)")
+    assert os.path.exists(path), f"{path} not built: run build() where the reference sources are"
     return path
 
 
@@ -48,15 +49,45 @@ def rms_lines(text):
     return vals
 
 
+def timing_table(text):
+    """Rows of op_timing_output's table (common/op_lib_core.c:381-403):
+    name -> (count, seconds, GB/s, GB/s or None)."""
+    rows = {}
+    lines = text.splitlines()
+    start = next(i for i, ln in enumerate(lines) if "kernel name" in ln) + 2
+    for ln in lines[start:]:
+        f = ln.split()
+        if len(f) == 5:
+            rows[f[4]] = (int(f[0]), float(f[1]), float(f[2]), float(f[3]))
+        elif len(f) == 4:
+            rows[f[3]] = (int(f[0]), float(f[1]), float(f[2]), None)
+        else:
+            break
+    return rows
+
+
 @pytest.mark.parametrize("prog", ["airfoil", "airfoil_op"])
 def test_unchanged_airfoil_program(tmp_path, golden_dir, prog):
     exe = need(prog)
     M.write_grid_dat(M.o_mesh(120, 60, "cartesian"), os.path.join(tmp_path, "new_grid.dat"))
-    got = rms_lines(run(exe, tmp_path))
+    out = run(exe, tmp_path)
+    got = rms_lines(out)
     ref = rms_lines(open(os.path.join(golden_dir, "airfoil_ref_120x60_cartesian.txt")).read())
     assert sorted(got) == sorted(ref) == list(range(100, 1001, 100))
     for it in ref:
         assert abs(got[it] - ref[it]) <= 2e-4 * abs(ref[it]), (it, got[it], ref[it])
+    # the program's closing op_timing_output(): one row per loop with the
+    # reference's columns; the loops through maps carry the second GB/s
+    table = timing_table(out)
+    print(f"[gpu] timing table {table}", flush=True)
+    counts = {"save_soln": 1000, "adt_calc": 2000, "res_calc": 2000, "bres_calc": 2000,
+              "update": 2000}
+    assert {k: v[0] for k, v in table.items()} == counts
+    for name, (_, secs, gbs, gbs2) in table.items():
+        assert secs > 0 and gbs > 0, name
+        assert (gbs2 is not None) == (name in ("adt_calc", "res_calc", "bres_calc")), name
+        if gbs2 is not None:
+            assert gbs2 >= gbs, name
 
 
 @pytest.mark.parametrize("prog", ["airfoil", "airfoil_op"])

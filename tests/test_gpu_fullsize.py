@@ -94,28 +94,64 @@ def test_random_ordering_2400x1200(renumber):
     assert rel < 1e-12 and relq < 1e-12
 
 
-def test_batch_64_meshes_fused_graphs():
-    """The benchmark's batch: 64 meshes of 10k-400k cells (7.12M cells) with
-    their own free streams, renumbered, 100 iterations in fused graphs of 10;
-    every mesh against the OpenMP oracle run of that mesh alone."""
-    note("batch FP64, 64 meshes / 7.12M cells, 100 iterations in fused graphs vs the oracle")
+@pytest.fixture(scope="module")
+def batch64():
+    """The benchmark's batch (64 meshes of 10k-400k cells, 7.12M cells, their
+    own free streams) and the OpenMP FP64 oracle run of each mesh alone, 100
+    iterations: (batch, [(rms history, q)] per mesh)."""
     batch = M.mesh_batch(nmesh=64, seed=1)
+    note("oracle OpenMP FP64 runs of the 64 meshes, 100 iterations each")
+    runs = []
+    for i in range(batch.nmesh):
+        st, oh, _ = O.airfoil_omp(batch.meshes[i], batch.consts[i], 100, "f64", nthreads=NTHR)
+        runs.append((oh, st.q.reshape(-1, 4).copy()))
+    return batch, runs
+
+
+def batch_run(batch, dtype):
     op2.op_init([], 0)
     try:
         for loop in ("res_calc", "bres_calc"):
             op2.set_loop_config(loop, op2.STRATEGY_OWNER_GATHER, 256, 0)
-        hist, q = AF.run_airfoil_batch(batch, 100, "double", graph=True, renumber=True,
-                                       graph_iters=10)
+        return AF.run_airfoil_batch(batch, 100, dtype, graph=True, renumber=True, graph_iters=10)
     finally:
         op2.op_exit()
         op2.set_loop_config(None, 0, 0, 0)
-    note("GPU done; oracle OpenMP runs")
-    worst = worst_q = 0.0
-    for i in range(batch.nmesh):
-        st, oh, _ = O.airfoil_omp(batch.meshes[i], batch.consts[i], 100, "f64", nthreads=NTHR)
-        qi = q[batch.cell_off[i]:batch.cell_off[i + 1]]
+
+
+def batch_errors(batch, runs, hist, q):
+    """Largest rms-history relative error and largest |q - q64| (absolute and
+    relative to the mesh's max |q|) over the meshes."""
+    worst = worst_q = worst_qrel = 0.0
+    for i, (oh, oq) in enumerate(runs):
+        qi = q[batch.cell_off[i]:batch.cell_off[i + 1]].astype(np.float64)
         worst = max(worst, float(np.max(np.abs(hist[:, i] - oh) / np.abs(oh))))
-        worst_q = max(worst_q,
-                      float(np.max(np.abs(qi - st.q.reshape(-1, 4))) / np.max(np.abs(st.q))))
+        dq = float(np.max(np.abs(qi - oq)))
+        worst_q = max(worst_q, dq)
+        worst_qrel = max(worst_qrel, dq / float(np.max(np.abs(oq))))
+    return worst, worst_q, worst_qrel
+
+
+def test_batch_64_meshes_fused_graphs(batch64):
+    """The benchmark's batch, renumbered, 100 iterations in fused graphs of
+    10; every mesh against the OpenMP oracle run of that mesh alone."""
+    batch, runs = batch64
+    note("batch FP64, 64 meshes / 7.12M cells, 100 iterations in fused graphs vs the oracle")
+    hist, q = batch_run(batch, "double")
+    worst, _, worst_q = batch_errors(batch, runs, hist, q)
     note(f"rms history max rel {worst:.2e}, q max rel {worst_q:.2e} over 64 meshes")
     assert worst < 1e-12 and worst_q < 1e-12
+
+
+def test_batch_64_meshes_fp32_form(batch64):
+    """The batch in the FP32 form (float state and flux, double update and
+    per-mesh rms), 100 iterations in fused graphs of 10: every mesh's rms
+    history within 1e-5 relative of its FP64 oracle run."""
+    batch, runs = batch64
+    note("batch FP32 form, 64 meshes / 7.12M cells, 100 iterations in fused graphs vs FP64")
+    hist, q = batch_run(batch, "float")
+    worst, worst_q, worst_qrel = batch_errors(batch, runs, hist, q)
+    note(f"rms history max rel {worst:.2e}, max |q - q64| {worst_q:.2e} "
+         f"({worst_qrel:.2e} of max |q|) over 64 meshes")
+    assert worst < 1e-5
+    assert worst_qrel < 1e-5

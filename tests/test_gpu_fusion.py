@@ -242,3 +242,52 @@ def test_fusion_switch_refused_during_capture(runtime):
     with pytest.raises(op2.OP2Error, match="graph capture"):
         op2.set_graph_fusion(False)
     op2.graph_end()
+
+
+def test_bres_calc_on_another_adt_is_not_fused(runtime):
+    """bres_calc reading an adt dat other than update's is not taken into the
+    update (the fused kernel applies the boundary flux with update's adt):
+    fused and unfused graphs give the same bits."""
+    note("fused graph with bres_calc on a second adt dat")
+    mesh = M.o_mesh(24, 12, "random", seed=4)
+    c = M.flow_constants().as_array()
+    out = []
+    for fuse in (True, False):
+        op2.op_exit()
+        op2.op_init([], 0)
+        op2.set_graph_fusion(fuse)
+        a = AF.Airfoil(mesh, c, "double")
+        adt2 = np.full(mesh.ncell, 3.0)
+        a.p_adt2 = op2.op_decl_dat(a.cells, 1, "double", adt2, "p_adt2")
+        base = a.bres_calc_args
+        a.bres_calc_args = lambda: base()[:3] + (
+            op2.op_arg_dat(a.p_adt2, 0, a.pbecell, 1, "double", op2.OP_READ),) + base()[4:]
+        a.iterate()
+        gid = a.capture_iteration(3)
+        a.run_graph(gid, 6)
+        op2.set_deferred_reductions(False)
+        q, _, adt, res = a.fetch()
+        out.append((q.copy(), adt.copy(), res.copy()))
+    op2.set_graph_fusion(True)
+    for x, y in zip(*out):
+        assert np.array_equal(x, y)
+
+
+def test_autotuning_refused_while_graphs_exist(runtime):
+    """Autotuning frees the plans of the losing configurations, which a
+    captured graph may use: both tuners refuse while a graph exists (the
+    graph is not replayed) and run again once it is destroyed."""
+    note("autotune after graph capture is refused")
+    mesh = M.o_mesh(24, 12, "random", seed=4)
+    c = M.flow_constants().as_array()
+    a = AF.Airfoil(mesh, c, "double")
+    a.iterate()
+    gid = a.capture_iteration(1)
+    with pytest.raises(op2.OP2Error, match="captured graphs exist"):
+        op2.autotune("res_calc", a.edges, a.res_calc_args(), 1)
+    with pytest.raises(op2.OP2Error, match="captured graphs exist"):
+        op2.autotune_online("res_calc", 1)
+    op2.graph_destroy(gid)
+    op2.set_deferred_reductions(False)
+    got = op2.autotune("res_calc", a.edges, a.res_calc_args(), 1)
+    assert got["strategy"] in range(1, 6)

@@ -280,3 +280,35 @@ def test_reductions_in_a_captured_graph(runtime):
     assert total[0] == 1.0 + ln.sum() or np.isclose(total[0], 1.0 + ln.sum(), rtol=1e-13)
     want_cnt = 5 * (np.bincount(e1, minlength=mesh.nnode) + np.bincount(e2, minlength=mesh.nnode))
     assert np.array_equal(op2.op_fetch_data(pc), want_cnt.astype(np.int32))
+
+
+def test_registered_loop_takes_precedence_over_built_in(runtime, loops):
+    """A loop registered under a built-in's name and argument signature runs
+    the registered kernel (here airfoil update's signature with half the
+    move); removing the registration brings the built-in kernel back."""
+    note("registered 'update' with another body runs instead of the built-in update")
+    from oplus2_matlab_amd import airfoil as AF
+
+    mesh = M.o_mesh(24, 12, "random", seed=2)
+    c = M.flow_constants().as_array()
+    rng = np.random.Generator(np.random.PCG64(8))
+    q0 = np.tile(c[6:10], (mesh.ncell, 1)) * (1 + 0.01 * rng.standard_normal((mesh.ncell, 4)))
+    res0 = 1e-3 * rng.standard_normal((mesh.ncell, 4))
+    a = AF.Airfoil(mesh, c, "double", q0=q0, res0=res0)
+    a.save_soln()
+    a.adt_calc()
+    adt = op2.op_fetch_data(a.p_adt).copy()
+    loops.testloops_register_half_update.argtypes = [C.c_int]
+    assert loops.testloops_register_half_update(1) == 0
+    try:
+        rms = a.update()
+    finally:
+        assert loops.testloops_register_half_update(0) == 0
+    q = op2.op_fetch_data(a.p_q).reshape(-1, 4)
+    half = 0.5 / adt[:, None] * res0
+    np.testing.assert_allclose(q, q0 - half, rtol=1e-15, atol=1e-15)
+    assert abs(rms - float((half ** 2).sum())) < 1e-12 * rms
+    assert not op2.op_fetch_data(a.p_res).any()
+    # the built-in update again: the full move (res is zero now: q = qold)
+    a.update()
+    assert np.array_equal(op2.op_fetch_data(a.p_q).reshape(-1, 4), q0)

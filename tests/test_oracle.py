@@ -231,6 +231,89 @@ def test_update_telescopes():
     assert abs(rms - ((qold - q) ** 2).sum()) < 1e-12 * rms
 
 
+def _perturbed_state(mesh, c, seed):
+    """float32 x, q (free stream +-2%), adt > 0 and res of one mesh."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    x = np.ascontiguousarray(mesh.x.astype(np.float32))
+    q = np.tile(c[6:10], (mesh.ncell, 1)) * (1 + 0.02 * rng.standard_normal((mesh.ncell, 4)))
+    adt = (0.5 + rng.random(mesh.ncell)) * 10.0
+    res = 1e-3 * rng.standard_normal((mesh.ncell, 4))
+    f32 = lambda a: np.ascontiguousarray(a.astype(np.float32))
+    return x, f32(q), f32(adt), f32(res)
+
+
+def test_mixed_form_flux_loops_are_the_f32_loops():
+    """adt_calc, res_calc and bres_calc of the precision form are the
+    reference's own float loops (airfoil/{adt_calc,res_calc,bres_calc}.h in
+    float): bit for bit the f32 oracle, which the reference build pins
+    above.  A flux term evaluated in double would change the bits."""
+    mesh = M.o_mesh(48, 24, "random", seed=4)
+    c = M.flow_constants().as_array()
+    x, q, adt0, res0 = _perturbed_state(mesh, c, 11)
+    out = {}
+    for form in ("f32", "mix"):
+        adt = np.zeros(mesh.ncell, np.float32)
+        O.adt_calc(form, mesh, x, q, adt, c)
+        res = res0.copy()
+        O.res_calc(form, mesh, x, q, adt0, res, c)
+        bres = res.copy()
+        O.bres_calc(form, mesh, x, q, adt0, bres, c)
+        out[form] = (adt, res, bres)
+    for a, b in zip(out["f32"], out["mix"]):
+        assert a.dtype == np.float32 and np.array_equal(a, b)
+    # the float loops differ from the double ones (so the pin is not vacuous)
+    adt64 = np.zeros(mesh.ncell)
+    O.adt_calc("f64", mesh, x.astype(np.float64), q.astype(np.float64), adt64, c)
+    assert not np.array_equal(adt64.astype(np.float32), out["mix"][0])
+
+
+def test_mixed_form_update_is_the_rounded_f64_update():
+    """update of the precision form: adti = 1/adt, del = adti res and
+    qold - del formed in double, q rounded once to float, rms summed in
+    double (airfoil/update.h:1-12).  Its q is float32 of the f64 update of
+    the same (exactly upcast) inputs and its rms is that update's rms, bit
+    for bit; an update carried out in float gives other bits."""
+    mesh = M.o_mesh(48, 24, "random", seed=4)
+    c = M.flow_constants().as_array()
+    _, qold, adt, res = _perturbed_state(mesh, c, 12)
+    res = np.ascontiguousarray(res * 50)  # moves of a few percent of q
+    q_mix = np.zeros_like(qold)
+    r_mix = res.copy()
+    rms_mix = O.update("mix", qold, q_mix, r_mix, adt)
+    q64 = np.zeros(qold.shape)
+    r64 = res.astype(np.float64)
+    rms64 = O.update("f64", qold.astype(np.float64), q64, r64, adt.astype(np.float64))
+    assert np.array_equal(q_mix, q64.astype(np.float32))
+    assert rms_mix == rms64
+    assert not r_mix.any()
+    q32 = np.zeros_like(qold)
+    rms32 = O.update("f32", qold, q32, res.copy(), adt)
+    assert np.count_nonzero(q32 != q_mix) > mesh.ncell // 10
+    assert rms32 != rms_mix
+
+
+def test_mixed_form_jacobi_update():
+    """jac1 update of the precision form: u += du + alpha r in float (the
+    f32 loop's bits) and u_sum, u_max accumulated in double: with du = r = 0
+    (u unchanged, exactly) the sums equal the f64 update's of the upcast u,
+    bit for bit, and differ from the float sums."""
+    rng = np.random.Generator(np.random.PCG64(13))
+    n = 4096
+    f32 = lambda a: np.ascontiguousarray(a, np.float32)
+    r, du, u = f32(rng.random(n)), f32(0.1 * rng.standard_normal(n)), f32(rng.random(n))
+    u_mix, u_32 = u.copy(), u.copy()
+    O.jac_update("mix", r, du.copy(), u_mix, 0.25)
+    O.jac_update("f32", r, du.copy(), u_32, 0.25)
+    assert np.array_equal(u_mix, u_32)
+    zero = np.zeros(n, np.float32)
+    s_mix = O.jac_update("mix", zero, zero.copy(), u_mix.copy(), 0.25)
+    s_64 = O.jac_update("f64", zero.astype(np.float64), np.zeros(n), u_mix.astype(np.float64),
+                        0.25)
+    s_32 = O.jac_update("f32", zero, zero.copy(), u_mix.copy(), 0.25)
+    assert s_mix == s_64
+    assert s_mix[0] != s_32[0]
+
+
 def test_mixed_form_close_to_f64():
     """The precision form (float state, double update) stays within float
     rounding of the double oracle over a short run."""

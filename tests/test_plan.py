@@ -133,12 +133,12 @@ def test_plan_is_cached(host_runtime):
     mesh = M.o_mesh(12, 6, "random", seed=1)
     d = declare(mesh)
     op2.plan_summary("res_calc", d["edges"], d["res_args"], op2.STRATEGY_COLOURED, 64)
-    L = op2.lib()
-    n0 = C.c_int.in_dll(L, "OP_plan_index").value
-    op2.plan_summary("res_calc", d["edges"], d["res_args"], op2.STRATEGY_COLOURED, 64)
-    assert C.c_int.in_dll(L, "OP_plan_index").value == n0
-    op2.plan_summary("res_calc", d["edges"], d["res_args"], op2.STRATEGY_ATOMIC, 64)
-    assert C.c_int.in_dll(L, "OP_plan_index").value == n0 + 1
+    n0 = op2.plan_summary("res_calc", d["edges"], d["res_args"], op2.STRATEGY_COLOURED,
+                          64)["cached_plans"]
+    s = op2.plan_summary("res_calc", d["edges"], d["res_args"], op2.STRATEGY_COLOURED, 64)
+    assert s["cached_plans"] == n0
+    s = op2.plan_summary("res_calc", d["edges"], d["res_args"], op2.STRATEGY_ATOMIC, 64)
+    assert s["cached_plans"] == n0 + 1
 
 
 @pytest.mark.parametrize("ordering", ["cartesian", "random", "rcm"])
@@ -252,3 +252,107 @@ def test_generic_plan_colours_every_indirect_write(host_runtime):
     # the same loop on the device kernels' strategies is refused (six pairs)
     with pytest.raises(op2.OP2Error, match="at most 4"):
         op2.plan_summary("gen_cell", d["cells"], args, op2.STRATEGY_COLOURED, 128)
+
+
+# ---------------------------------------------------------------------------
+# op_plan_core / op_plan_get: the plan in the reference's layout
+
+def _view_args(mesh, prob, rb):
+    """(name, set, args, inds) of the reference's four indirect loops, with
+    the inds of airfoil/<loop>_kernel.cpp and jac1/res_kernel.cpp, on dats
+    of rb-byte reals."""
+    t = "double" if rb == 8 else "float"
+    real = np.float64 if rb == 8 else np.float32
+    d = declare(mesh)
+    x = op2.op_decl_dat_host(d["nodes"], 2, t, mesh.x.astype(real), "x")
+    q = op2.op_decl_dat_host(d["cells"], 4, t, np.zeros(4 * mesh.ncell, real), "q")
+    adt = op2.op_decl_dat_host(d["cells"], 1, t, np.zeros(mesh.ncell, real), "adt")
+    res = op2.op_decl_dat_host(d["cells"], 4, t, np.zeros(4 * mesh.ncell, real), "res")
+    bound = op2.op_decl_dat_host(d["bedges"], 1, "int", mesh.bound, "bound")
+    R, W, I = op2.OP_READ, op2.OP_WRITE, op2.OP_INC
+    A = op2.op_arg_dat
+    adt_args = [A(x, k, d["pcell"], 2, t, R) for k in range(4)] + [
+        A(q, -1, None, 4, t, R), A(adt, -1, None, 1, t, W)]
+    res_args = [A(x, 0, d["pedge"], 2, t, R), A(x, 1, d["pedge"], 2, t, R),
+                A(q, 0, d["pecell"], 4, t, R), A(q, 1, d["pecell"], 4, t, R),
+                A(adt, 0, d["pecell"], 1, t, R), A(adt, 1, d["pecell"], 1, t, R),
+                A(res, 0, d["pecell"], 4, t, I), A(res, 1, d["pecell"], 4, t, I)]
+    bres_args = [A(x, 0, d["pbedge"], 2, t, R), A(x, 1, d["pbedge"], 2, t, R),
+                 A(q, 0, d["pbecell"], 4, t, R), A(adt, 0, d["pbecell"], 1, t, R),
+                 A(res, 0, d["pbecell"], 4, t, I), A(bound, -1, None, 1, "int", R)]
+    jn = op2.op_decl_set(prob.nnode, "jnodes")
+    je = op2.op_decl_set(prob.nedge, "jedges")
+    pp = op2.op_decl_map(je, jn, 2, prob.pp, "pp")
+    jA = op2.op_decl_dat_host(je, 1, t, prob.A.astype(real), "A")
+    u = op2.op_decl_dat_host(jn, 1, t, np.zeros(prob.nnode, real), "u")
+    du = op2.op_decl_dat_host(jn, 1, t, np.zeros(prob.nnode, real), "du")
+    beta = np.zeros(1, real)
+    jac_args = [A(jA, -1, None, 1, t, R), A(u, 1, pp, 1, t, R), A(du, 0, pp, 1, t, I),
+                op2.op_arg_gbl(beta, 1, t, R)]
+    return [("adt_calc", d["cells"], adt_args, [0, 0, 0, 0, -1, -1]),
+            ("res_calc", d["edges"], res_args, [0, 0, 1, 1, 2, 2, 3, 3]),
+            ("bres_calc", d["bedges"], bres_args, [0, 0, 1, 2, 3, -1]),
+            ("jac_res", je, jac_args, [-1, 0, 1, -1])]
+
+
+@pytest.mark.parametrize("rb", [4, 8])
+def test_op_plan_get_matches_reference_plans(host_runtime, golden_dir, rb):
+    """op_plan_core through the C ABI returns, for adt_calc, res_calc,
+    bres_calc and jac1 res (part sizes 0 and 64), the plan the reference's
+    own op_plan_core built for the same mesh (tests/golden fixtures from the
+    reference build): every array and the float byte model bit for bit, and
+    the oracle's restated op_plan_check finds no error in it."""
+    import os
+
+    from oracle import oracle as O
+
+    fx = np.load(os.path.join(golden_dir, "plans_24x12_random_s1.npz"))
+    mesh = M.o_mesh(24, 12, "random", seed=1)
+    prob = M.jacobi_grid(16, seed=1)
+    for name, set_, args, inds in _view_args(mesh, prob, rb):
+        for part in (0, 64):
+            pre = f"r{rb}_p{part}_{name}_"
+            v = op2.op_plan_get(name, set_, args, inds, part)
+            nb, nc, nsh = fx[pre + "header"][:3]
+            assert (v["nblocks"], v["ncolors"], v["nshared"]) == (nb, nc, nsh), pre
+            assert np.array_equal(np.array([v["transfer"], v["transfer2"]], np.float32),
+                                  fx[pre + "transfer"]), pre
+            for k in ("nthrcol", "thrcol", "offset", "nelems", "ncolblk", "blkmap", "ind_offs",
+                      "ind_sizes", "nindirect"):
+                assert np.array_equal(v[k], fx[pre + k]), pre + k
+            ninds = max(inds) + 1
+            for m in range(ninds):
+                assert np.array_equal(v["ind_maps"][m], fx[pre + f"ind_map{m}"]), pre
+            for a, lm in enumerate(v["loc_maps"]):
+                assert (lm is None) == (inds[a] < 0)
+                if lm is not None:
+                    assert np.array_equal(lm, fx[pre + f"loc_map{a}"]), pre + str(a)
+            assert list(v["idxs"]) == [a.idx for a in args]
+            assert list(v["accs"]) == [a.acc for a in args]
+            if name == "jac_res":
+                _, oargs, _ = O.jacobi_plan_args(prob, rb)
+            else:
+                _, oargs, _ = O.airfoil_plan_args(mesh, name, rb)
+            assert O.check_plan(set_.size, oargs, ninds, v) == 0, pre
+
+
+def test_op_plan_get_cache_and_globals(host_runtime):
+    """A second call with the same inputs returns the same plan (count 2);
+    OP_plans / OP_plan_index list the views; invalid inds are an error."""
+    mesh = M.o_mesh(12, 6, "random", seed=1)
+    views = _view_args(mesh, M.jacobi_grid(8, seed=1), 8)
+    name, set_, args, inds = views[1]
+    L = op2.lib()
+    p1 = op2.op_plan_get(name, set_, args, inds, 64)
+    p2 = op2.op_plan_get(name, set_, args, inds, 64)
+    assert p2["count"] == 2 and p1["count"] == 1
+    assert C.c_int.in_dll(L, "OP_plan_index").value == 1
+    op2.op_plan_get(name, set_, args, inds, 32)
+    assert C.c_int.in_dll(L, "OP_plan_index").value == 2
+    plans = C.POINTER(op2.OpPlan).in_dll(L, "OP_plans")
+    assert plans[0].nblocks == p1["nblocks"] and plans[0].count == 2
+    assert plans[1].part_size == 32
+    bad = list(inds)
+    bad[2] = -2
+    with pytest.raises(op2.OP2Error, match="out of range"):
+        op2.op_plan_get(name, set_, args, bad, 64)
