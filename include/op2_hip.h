@@ -317,17 +317,25 @@ int op2_graph_end(void);
 int op2_graph_launch(int id, int count);
 int op2_graph_destroy(int id);
 
-/* Loop fusion during graph capture (on by default).  A captured bres_calc
- * joins the update of the cells whose res it increments; an update joins the
+/* Loop fusion during graph capture.  Level 1: a captured bres_calc joins
+ * the update of the cells whose res it increments; an update joins the
  * save_soln that copies its q and the adt_calc that reads it, so q is read
  * once; a save_soln alone joins that adt_calc; the rms sum of an update is
  * folded by the next res_calc's kernel; and a res that an earlier captured
  * update zeroed is taken as zero (not read, and not stored when the next
- * owner-compute res_calc overwrites every cell).  Results are the same bit
- * for bit.  Fusion sees only op_par_loop calls: other work captured on
- * op2_stream that writes a dat calls for fusion off during that capture.
- * Returns the previous setting, or -1 during a capture (no change). */
-int op2_set_graph_fusion(int enable);
+ * owner-compute res_calc overwrites every cell).  Level 2 (the default)
+ * adds one kernel per Runge-Kutta stage: a res_calc on an owner-gather plan
+ * whose tiles own every cell, the bres_calc, update, next save_soln and
+ * adt_calc that follow on the same cells and dats run in one kernel, each
+ * tile finishing the cells it owns (res is not stored between stages; q and
+ * adt alternate between the dat and a second buffer the runtime allocates
+ * at op2_graph_begin, and are back in the dat at the end of the graph).
+ * q, adt and res keep the bits of the separate loops; at level 2 the rms
+ * sums are grouped per tile, so they agree to double rounding.  Fusion sees
+ * only op_par_loop calls: other work captured on op2_stream that reads or
+ * writes a dat calls for fusion off during that capture.  Returns the
+ * previous level, or -1 during a capture (no change). */
+int op2_set_graph_fusion(int level);
 
 /* Device pointer of a dat (for zero-copy interop). */
 void *op2_dat_device_ptr(op_dat dat);

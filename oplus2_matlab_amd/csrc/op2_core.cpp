@@ -737,6 +737,7 @@ int op2_graph_begin(void) {
            "before graph capture", o.first.c_str());
   timing_drain();
   reduct_scratch(256 + sizeof(double) * 2048 * 4);
+  prepare_capture();
   OP2_HIP_CHECK(hipStreamSynchronize(r.stream));
   OP2_HIP_CHECK(hipStreamBeginCapture(r.stream, hipStreamCaptureModeThreadLocal));
   r.capturing = true;
@@ -767,11 +768,11 @@ int op2_graph_end(void) {
   API_END(-1)
 }
 
-int op2_set_graph_fusion(int enable) {
+int op2_set_graph_fusion(int level) {
   Runtime &r = rt();
   const int prev = r.graph_fusion;
   if (r.capturing) return -1;
-  r.graph_fusion = enable != 0;
+  r.graph_fusion = level <= 0 ? 0 : level == 1 ? 1 : 2;
   return prev;
 }
 

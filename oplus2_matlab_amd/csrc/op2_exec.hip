@@ -870,6 +870,155 @@ __global__ __launch_bounds__(NT) void k_update_seg(const T *qold, T *q,
   }
 }
 
+// One Runge-Kutta stage of a captured graph as one kernel: res_calc,
+// bres_calc, update, and optionally the next iteration's save_soln and the
+// adt_calc that follows, on owner-gather tiles that own every cell.  A tile
+// holds the complete edge sum of each cell it owns, so it finishes the cell
+// in the same workgroup and res never reaches memory.  Other tiles still
+// read the cell's old q and adt as halo, so the new q and adt go to other
+// buffers (q_out, adt_out; the runtime alternates two buffers per dat).
+// Per owned cell, in the order the separate loops use:
+//   r = res (or +0 when res is known to hold zeros) + the tile's edge fluxes
+//       in the plan's fixed order (res_calc)
+//   r += each boundary flux, in edge order (bres_calc)
+//   del = r / adt in double, q_out = qold - del rounded once, rms += del^2
+//       (update); res = 0 unless the next stage takes it as zero
+//   SAVE: qold_out = q_out (save_soln of the next iteration)
+//   ADT: adt_out = adt of q_out (adt_calc), else adt_out = adt
+// so q, adt and res hold the bits of the separate loops.  The rms sum of the
+// tile's cells is one partial per tile (folded by the next kernel); in the
+// batched form every tile lies in one mesh and the tiles of mesh m are
+// folded in order.
+template <typename T>
+struct StageArgs {
+  const int *blk_off;
+  int nblk;
+  const int4 *ids;
+  const int *owned_off, *owned;
+  const ushort4 *ell;
+  const T *x, *q_in, *adt_in, *qold;
+  T *res, *q_out, *adt_out, *qold_out;
+  const int4 *pcell;  // ADT: the cell -> node map of adt_calc
+  const T *xc;        // ADT: the node coordinates adt_calc reads
+  double *partials;   // one per tile
+  unsigned *log_count, *log_base;  // batched form: log slots of this stage's nseg results
+  int nseg;
+};
+
+template <typename T, bool OWNID, bool BRES, bool SAVE, bool ADT>
+__global__ __launch_bounds__(NT) void k_stage(StageArgs<T> s, BresIdx<T> bi, AirfoilConsts<T> k,
+                                              CarriedFold fold, bool zero_res, bool store_res) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ double red[NT / kWave];
+  if (carries_fold(fold)) carried_fold(fold, red);
+  if (s.log_count && blockIdx.x == 0 && threadIdx.x == 0)
+    *s.log_base = atomicAdd(s.log_count, (unsigned)s.nseg);
+  const int b = xcd_block(blockIdx.x, s.nblk);
+  const int p0 = s.blk_off[b], np = s.blk_off[b + 1] - p0;
+  const int o0 = s.owned_off[b], no = s.owned_off[b + 1] - o0;
+  T *fs = reinterpret_cast<T *>(smem);  // [4][np]
+  const int tid = threadIdx.x;
+  const bool own = tid < no;
+  const int c = own ? (OWNID ? o0 + tid : s.owned[o0 + tid]) : 0;
+  // the owned cell's records, issued before the flux phase so that their
+  // latency overlaps it
+  ushort4 e4 = make_ushort4(0xffff, 0xffff, 0xffff, 0xffff);
+  T rpre[4] = {T(0), T(0), T(0), T(0)}, qo[4], a = T(1);
+  int4 nd = make_int4(0, 0, 0, 0);
+  int bidx = -1;
+  if (own) {
+    const op2_v4us v = __builtin_nontemporal_load((const op2_v4us *)(s.ell + o0 + tid));
+    e4 = make_ushort4(v.x, v.y, v.z, v.w);
+    if (!zero_res) Vec<T>::load4(s.res + 4 * (size_t)c, rpre);
+    Vec<T>::load4(s.qold + 4 * (size_t)c, qo);
+    a = s.adt_in[c];
+    if (ADT) nd = s.pcell[c];
+    if (BRES) {
+      const uint2 w = bi.word[c >> 5];
+      const unsigned bit = 1u << (c & 31);
+      bidx = (w.x & bit) ? (int)(w.y + __popc(w.x & (bit - 1u))) : -1;
+    }
+  }
+  for (int p = tid; p < np; p += NT) {
+    const op2_v4i g = __builtin_nontemporal_load((const op2_v4i *)(s.ids + p0 + p));
+    // lower-numbered node and cell first (see k_res_calc_owner_direct)
+    const int na = min(g.x, g.y), nb = max(g.x, g.y);
+    const int ca = min(g.z, g.w), cb = max(g.z, g.w);
+    T xa[2], xb[2], qa[4], qb[4], f[4];
+    Vec<T>::load2(s.x + 2 * (size_t)na, xa);
+    Vec<T>::load2(s.x + 2 * (size_t)nb, xb);
+    Vec<T>::load4(s.q_in + 4 * (size_t)ca, qa);
+    Vec<T>::load4(s.q_in + 4 * (size_t)cb, qb);
+    const T aa = s.adt_in[ca], ab = s.adt_in[cb];
+    const bool sn = g.x != na, sc = g.z != ca;
+    T x1[2], x2[2], q1[4], q2[4];
+#pragma unroll
+    for (int d = 0; d < 2; d++) {
+      x1[d] = sn ? xb[d] : xa[d];
+      x2[d] = sn ? xa[d] : xb[d];
+    }
+#pragma unroll
+    for (int d = 0; d < 4; d++) {
+      q1[d] = sc ? qb[d] : qa[d];
+      q2[d] = sc ? qa[d] : qb[d];
+    }
+    edge_flux(x1, x2, q1, q2, sc ? ab : aa, sc ? aa : ab, k, f);
+#pragma unroll
+    for (int d = 0; d < 4; d++) fs[d * np + p] = f[d];
+  }
+  // the nodes of the owned cell for its new adt: gathered while the other
+  // waves finish their fluxes
+  T xs[4][2];
+  if (ADT && own) {
+    Vec<T>::load2(s.xc + 2 * (size_t)nd.x, xs[0]);
+    Vec<T>::load2(s.xc + 2 * (size_t)nd.y, xs[1]);
+    Vec<T>::load2(s.xc + 2 * (size_t)nd.z, xs[2]);
+    Vec<T>::load2(s.xc + 2 * (size_t)nd.w, xs[3]);
+  }
+  __syncthreads();
+  double acc = 0.0;
+  if (own) {
+    T sum[4] = {T(0), T(0), T(0), T(0)};
+    ell_sum(fs, np, e4, sum);
+    T r[4];
+#pragma unroll
+    for (int d = 0; d < 4; d++) r[d] = rpre[d] + sum[d];
+    if (BRES && bidx >= 0) {
+      T qc[4];
+      Vec<T>::load4(s.q_in + 4 * (size_t)c, qc);
+      const int4 rec[2] = {bi.rec[2 * bidx], bi.rec[2 * bidx + 1]};
+#pragma unroll
+      for (int j = 0; j < 2; j++) {
+        if (rec[j].z < 0) break;
+        T x1[2], x2[2], f[4];
+        Vec<T>::load2(bi.x + 2 * (size_t)rec[j].x, x1);
+        Vec<T>::load2(bi.x + 2 * (size_t)rec[j].y, x2);
+        const T *qi = bi.qinf_dat ? bi.qinf_dat + 4 * (size_t)rec[j].w : k.qinf;
+        boundary_flux(x1, x2, qc, a, bi.bound[rec[j].z], qi, k, f);
+#pragma unroll
+        for (int d = 0; d < 4; d++) r[d] += f[d];
+      }
+    }
+    const double adti = 1.0 / (double)a;
+    T qn[4];
+#pragma unroll
+    for (int d = 0; d < 4; d++) {
+      const double del = adti * (double)r[d];
+      qn[d] = (T)((double)qo[d] - del);
+      acc += del * del;
+    }
+    Vec<T>::store4(s.q_out + 4 * (size_t)c, qn);
+    if (SAVE) Vec<T>::store4(s.qold_out + 4 * (size_t)c, qn);
+    s.adt_out[c] = ADT ? adt_of(xs[0], xs[1], xs[2], xs[3], qn, k) : a;
+    if (store_res) {
+      const T z[4] = {T(0), T(0), T(0), T(0)};
+      Vec<T>::store4(s.res + 4 * (size_t)c, z);
+    }
+  }
+  acc = block_reduce<kRedSum, NT>(acc, red);
+  if (tid == 0) s.partials[b] = acc;
+}
+
 // One workgroup per mesh m: out = sum of its partials (thread-strided, then
 // the block tree: a fixed order); the dat gets dat[m] += out and, with
 // deferred reductions, the log slot reserved by k_update_seg gets out (the
@@ -1355,6 +1504,7 @@ void run_res_calc(const char *name, op_set set, int nargs, op_arg *a) {
   }
   if (strategy == OP2_STRATEGY_OWNER_GATHER) {
     Plan *p = plan_for(name, set, nargs, a, strategy, part_of(name, 256));
+    p->airfoil_res = true;  // a stage kernel of a capture may run on it
     const size_t lds = sizeof(T) * 4 * (size_t)p->max_positions;
     const int own = (p->max_owned + NT - 1) / NT;
     if (own > 4) fail("op2: owner tile of %s exceeds %d targets", name, 4 * NT);
@@ -1664,6 +1814,70 @@ SegPlan &seg_plan(op_map m) {
 
 // Frees the cached segment plans and boundary indexes (their maps are about
 // to be freed).
+namespace {
+
+// Buffers of the stage kernels (k_stage) on one owner-gather res_calc plan,
+// made before a capture because a capture cannot allocate: two sets of
+// per-tile rms partials and of batched log slots (a stage writes one set
+// while its workgroups fold the previous stage's), the single form's result
+// slot, and for each cell -> mesh map of a batched update the first tile of
+// each mesh (empty when a tile holds cells of two meshes).
+struct StageBufs {
+  const Plan *plan = nullptr;
+  double *partials[2] = {nullptr, nullptr};
+  unsigned *log_base[2] = {nullptr, nullptr};
+  double *out = nullptr;
+  int parity = 0;
+  std::vector<std::pair<op_map, int *>> tile_off;
+};
+
+std::vector<StageBufs> &stage_bufs() {
+  static std::vector<StageBufs> v;
+  return v;
+}
+
+StageBufs *bufs_of(const Plan *p) {
+  for (auto &b : stage_bufs())
+    if (b.plan == p) return &b;
+  return nullptr;
+}
+
+const int *tiles_of_meshes(const StageBufs &b, op_map m) {
+  for (auto &t : b.tile_off)
+    if (t.first == m) return t.second;
+  return nullptr;
+}
+
+// The second buffer of every dat a stage writes (q, adt), and which of the
+// two holds the dat's current values during a capture (true: the second).
+std::map<op_dat, void *> &shadow_bufs() {
+  static std::map<op_dat, void *> m;
+  return m;
+}
+
+std::map<op_dat, bool> &in_shadow() {
+  static std::map<op_dat, bool> m;
+  return m;
+}
+
+void free_stage_bufs(StageBufs &b, std::vector<void *> *keep) {
+  auto drop = [keep](void *p) {
+    if (!p) return;
+    if (keep)
+      keep->push_back(p);
+    else
+      dev_free(p);
+  };
+  for (int i = 0; i < 2; i++) {
+    drop(b.partials[i]);
+    drop(b.log_base[i]);
+  }
+  drop(b.out);
+  for (auto &t : b.tile_off) drop(t.second);
+}
+
+}  // namespace
+
 void release_exec_caches() {
   for (auto &sp : seg_cache()) {
     dev_free(sp.d_seg_off);
@@ -1677,6 +1891,81 @@ void release_exec_caches() {
     dev_free(b.d_rec);
   }
   bres_cache().clear();
+  for (auto &b : stage_bufs()) free_stage_bufs(b, nullptr);
+  stage_bufs().clear();
+  for (auto &s : shadow_bufs()) dev_free(s.second);
+  shadow_bufs().clear();
+  in_shadow().clear();
+}
+
+void forget_plan_caches(const Plan *p) {
+  auto &v = stage_bufs();
+  for (size_t i = 0; i < v.size(); i++)
+    if (v[i].plan == p) {
+      free_stage_bufs(v[i], live_graphs() ? &rt().retired : nullptr);
+      v.erase(v.begin() + i);
+      return;
+    }
+}
+
+// Before a capture with stage fusion: for every res_calc plan of the
+// owner-gather strategy whose tiles own every cell, the stage buffers, the
+// second buffers of its q and adt dats, and the tiles of each mesh of every
+// batched update's cell -> mesh map.
+void prepare_capture() {
+  Runtime &r = rt();
+  for (auto &kv : in_shadow()) kv.second = false;
+  if (r.graph_fusion < 2) return;
+  for (auto &up : r.plans) {
+    const Plan *p = up.get();
+    if (!p->airfoil_res || p->strategy != OP2_STRATEGY_OWNER_GATHER || !p->d_ell ||
+        p->max_owned > NT)
+      continue;
+    const op_set cells = p->groups[p->owner_group].map->to;
+    if ((int)p->owned.size() != cells->size) continue;
+    StageBufs *b = bufs_of(p);
+    if (!b) {
+      StageBufs nb;
+      nb.plan = p;
+      for (int i = 0; i < 2; i++) {
+        nb.partials[i] = (double *)dev_alloc(sizeof(double) * std::max(p->nblocks, 1));
+        nb.log_base[i] = (unsigned *)dev_alloc(256);
+      }
+      nb.out = (double *)dev_alloc(256);
+      stage_bufs().push_back(nb);
+      b = &stage_bufs().back();
+    }
+    for (int a : {2, 4}) {  // q and adt
+      op_dat d = p->dats[a];
+      if (!shadow_bufs().count(d))
+        shadow_bufs()[d] = dev_alloc((size_t)d->size * d->set->size);
+      in_shadow()[d] = false;
+    }
+    for (auto &sp : seg_cache()) {
+      bool known = false;
+      for (auto &t : b->tile_off) known |= t.first == sp.map;
+      if (sp.map->from != cells || known) continue;
+      const int *mm = map_host(sp.map);
+      std::vector<int> first(sp.nseg + 1, p->nblocks);
+      bool ok = true;
+      int last = -1;
+      for (int t = 0; t < p->nblocks && ok; t++) {
+        const int lo = p->owned_off[t], hi = p->owned_off[t + 1];
+        const int m = mm[p->owned[lo]];
+        for (int i = lo; i < hi && ok; i++) ok = mm[p->owned[i]] == m;
+        ok = ok && m >= last;
+        if (m != last)
+          for (int k = last + 1; k <= m; k++) first[k] = t;
+        last = m;
+      }
+      int *d = nullptr;
+      if (ok) {
+        d = (int *)dev_alloc(sizeof(int) * first.size());
+        h2d(d, first.data(), sizeof(int) * first.size());
+      }
+      b->tile_off.emplace_back(sp.map, d);
+    }
+  }
 }
 
 namespace {
@@ -2112,14 +2401,246 @@ bool fuses_with_held(Loop l, int rb, op_set set, const op_arg *a) {
   const op_dat q = h.l == Loop::kSaveSoln ? h.args[0].dat : h.args[1].dat;
   return a[4].dat == q;
 }
+
+// ---- one kernel per Runge-Kutta stage (graph fusion level 2) ----------------
+//
+// During a capture a res_calc on an owner-gather plan whose tiles own every
+// cell is held as the start of a stage; the bres_calc, the update and the
+// next save_soln and adt_calc on the same cells and dats join it, and the
+// stage runs as one k_stage kernel when the next call does not join.  The
+// stage reads q and adt from the buffer that holds their current values and
+// writes them to the other one (two stages per airfoil iteration bring them
+// back); any other loop first copies such dats back (sync_shadows).  When
+// the call that ends a stage starts the next stage on the same res, the
+// stage does not store res's zeros and the next one takes res as zero.
+
+struct Stage {
+  bool active = false;
+  int rb = 0;
+  Plan *plan = nullptr;
+  std::string name;
+  op_set eset = nullptr;
+  std::vector<op_arg> res, bres, upd, save, adt;
+  bool batch = false;
+  bool zero_in = false;  // res holds zeros at the stage's start
+};
+
+Stage &stage() {
+  static Stage s;
+  return s;
+}
+
+char *live_ptr(op_dat d) {
+  auto it = in_shadow().find(d);
+  return it != in_shadow().end() && it->second ? (char *)shadow_bufs()[d] : d->data_d;
+}
+
+char *other_ptr(op_dat d) {
+  return in_shadow()[d] ? d->data_d : (char *)shadow_bufs().at(d);
+}
+
+template <typename T>
+__global__ __launch_bounds__(NT) void k_copy_elems(const T *__restrict__ src, T *__restrict__ dst,
+                                                   long long n) {
+  for (long long i = (long long)blockIdx.x * NT + threadIdx.x; i < n;
+       i += (long long)gridDim.x * NT)
+    dst[i] = src[i];
+}
+
+// Copies every dat whose current values are in its second buffer back to its
+// own (before a loop other than a stage reads or writes dats, and at the end
+// of a capture).
+void sync_shadows() {
+  for (auto &kv : in_shadow()) {
+    if (!kv.second) continue;
+    op_dat d = kv.first;
+    const long long words = (long long)d->size * d->set->size / 4;
+    hipLaunchKernelGGL(k_copy_elems<unsigned>, dim3(grid_for(words, 1 << 16)), dim3(NT), 0,
+                       rt().stream, (const unsigned *)shadow_bufs()[d], (unsigned *)d->data_d,
+                       words);
+    check_launch("graph stage");
+    kv.second = false;
+  }
+}
+
+// The owner-gather plan of a res_calc call that can start a stage, else null.
+Plan *starts_stage(const char *name, Loop l, int rb, op_set set, int nargs, op_arg *a) {
+  const Runtime &r = rt();
+  if (r.graph_fusion < 2 || l != Loop::kResCalc) return nullptr;
+  auto ot = r.online.find(name);
+  if (ot != r.online.end() && !ot->second.done) return nullptr;
+  if (strategy_of(name) != OP2_STRATEGY_OWNER_GATHER) return nullptr;
+  Plan *p = plan_for(name, set, nargs, a, OP2_STRATEGY_OWNER_GATHER, part_of(name, 256));
+  if (!bufs_of(p) || !shadow_bufs().count(a[2].dat) || !shadow_bufs().count(a[4].dat))
+    return nullptr;
+  const op_dat q = a[2].dat, adt = a[4].dat, res = a[6].dat;
+  if (q == adt || q == res || adt == res || a[0].dat == q || a[0].dat == adt) return nullptr;
+  (void)rb;
+  return p;
+}
+
+// A call that joins the held stage (returns true).
+bool stage_takes(Loop l, int rb, op_set set, int nargs, op_arg *a) {
+  Stage &st = stage();
+  if (!st.active || rb != st.rb) return false;
+  const op_dat q = st.res[2].dat, adt = st.res[4].dat, res = st.res[6].dat;
+  const op_set cells = st.res[6].map->to;
+  if (l == Loop::kBresCalc && st.bres.empty() && st.upd.empty()) {
+    const BresIndex *ix = find_bres_index(nargs, a);
+    if (!ix || !ix->usable || a[2].map->to != cells || a[4].dat != res || a[2].dat != q ||
+        a[3].dat != adt || a[0].dat == q || a[0].dat == adt)
+      return false;
+    st.bres.assign(a, a + nargs);
+    return true;
+  }
+  if ((l == Loop::kUpdate || l == Loop::kUpdateBatch) && st.upd.empty()) {
+    if (set != cells || a[2].dat != res || a[1].dat != q || a[3].dat != adt)
+      return false;
+    if (a[0].dat == q || a[0].dat == adt || a[0].dat == res) return false;
+    if (l == Loop::kUpdateBatch && !tiles_of_meshes(*bufs_of(st.plan), a[4].map)) return false;
+    st.upd.assign(a, a + nargs);
+    st.batch = l == Loop::kUpdateBatch;
+    return true;
+  }
+  if (l == Loop::kSaveSoln && !st.upd.empty() && st.save.empty() && st.adt.empty()) {
+    if (set != cells || a[0].dat != q || a[1].dat == q || a[1].dat == adt || a[1].dat == res)
+      return false;
+    st.save.assign(a, a + nargs);
+    return true;
+  }
+  if (l == Loop::kAdtCalc && !st.upd.empty() && st.adt.empty()) {
+    if (set != cells || a[4].dat != q || a[5].dat != adt || a[0].dat == q || a[0].dat == adt)
+      return false;
+    st.adt.assign(a, a + nargs);
+    return true;
+  }
+  return false;
+}
+
+#define OP2_STG(O, B, S, A) k_stage<T, O, B, S, A>
+template <typename T>
+auto stage_kernel(bool oid, bool bres, bool save, bool adt) {
+  decltype(&OP2_STG(true, true, true, true)) table[16] = {
+      OP2_STG(false, false, false, false), OP2_STG(false, false, false, true),
+      OP2_STG(false, false, true, false),  OP2_STG(false, false, true, true),
+      OP2_STG(false, true, false, false),  OP2_STG(false, true, false, true),
+      OP2_STG(false, true, true, false),   OP2_STG(false, true, true, true),
+      OP2_STG(true, false, false, false),  OP2_STG(true, false, false, true),
+      OP2_STG(true, false, true, false),   OP2_STG(true, false, true, true),
+      OP2_STG(true, true, false, false),   OP2_STG(true, true, false, true),
+      OP2_STG(true, true, true, false),    OP2_STG(true, true, true, true)};
+  return table[oid * 8 + bres * 4 + save * 2 + adt];
+}
+#undef OP2_STG
+
+template <typename T>
+void run_stage(Stage &st, bool store_res) {
+  Runtime &r = rt();
+  Plan *p = st.plan;
+  StageBufs *sb = bufs_of(p);
+  const op_dat q = st.res[2].dat, adt = st.res[4].dat, res = st.res[6].dat;
+  const bool save = !st.save.empty(), has_adt = !st.adt.empty(), has_bres = !st.bres.empty();
+  StageArgs<T> a{};
+  a.blk_off = p->d_blk_off;
+  a.nblk = p->nblocks;
+  a.ids = (const int4 *)p->d_ids;
+  a.owned_off = p->d_owned_off;
+  a.owned = p->d_owned;
+  a.ell = (const ushort4 *)p->d_ell;
+  a.x = (const T *)live_ptr(st.res[0].dat);
+  a.q_in = (const T *)live_ptr(q);
+  a.adt_in = (const T *)live_ptr(adt);
+  a.qold = (const T *)live_ptr(st.upd[0].dat);
+  a.res = (T *)res->data_d;
+  a.q_out = (T *)other_ptr(q);
+  a.adt_out = (T *)other_ptr(adt);
+  a.qold_out = save ? (T *)st.save[1].dat->data_d : nullptr;
+  a.pcell = has_adt ? (const int4 *)st.adt[0].map->map_d : nullptr;
+  a.xc = has_adt ? (const T *)live_ptr(st.adt[0].dat) : nullptr;
+  const int par = sb->parity;
+  sb->parity ^= 1;
+  a.partials = sb->partials[par];
+  a.log_count = st.batch && r.deferred ? r.d_log_count : nullptr;
+  a.log_base = sb->log_base[par];
+  const int *tiles = st.batch ? tiles_of_meshes(*sb, st.upd[4].map) : nullptr;
+  a.nseg = st.batch ? st.upd[4].map->to->size : 0;
+  BresIdx<T> bi = has_bres ? bres_idx<T>(st.bres.data(), (int)st.bres.size(), false) : BresIdx<T>{};
+  auto kern = stage_kernel<T>(p->owned_identity, has_bres, save, has_adt);
+  const size_t lds = std::max(sizeof(T) * 4 * (size_t)p->max_positions, (size_t)64);
+  allow_lds(kern, lds);
+  const CarriedFold fold = take_fold(p->nblocks);
+  hipLaunchKernelGGL(kern, dim3(p->nblocks), dim3(NT), lds, r.stream, a, bi, airfoil_consts<T>(),
+                     fold, st.zero_in, store_res);
+  check_launch("graph stage");
+  in_shadow()[q] = !in_shadow()[q];
+  in_shadow()[adt] = !in_shadow()[adt];
+  CarriedFold f{};
+  f.partials = sb->partials[par];
+  f.log = r.deferred ? r.d_log : nullptr;
+  if (st.batch) {
+    f.pslot = tiles;
+    f.nseg = a.nseg;
+    f.dat = (double *)st.upd[4].dat->data_d;
+    f.log_base = sb->log_base[par];
+  } else {
+    f.n = p->nblocks;
+    f.out = sb->out;
+    f.log_count = r.d_log_count;
+  }
+  if (f.nseg > 0 || f.n > 0) pending_fold() = PendingFold{true, f};
+  zero_dats().erase(q);
+  zero_dats().erase(adt);
+  if (save) zero_dats().erase(st.save[1].dat);
+  zero_dats().insert(res);
+}
+
+// Launches the held stage: as one k_stage kernel once an update has joined,
+// else as the separate res_calc (and bres_calc) kernels.  next_takes_res:
+// the call that ends the stage starts a stage on the same res (which will
+// take it as zero), so the zeros need not be stored.
+void launch_stage(bool next_takes_res) {
+  Stage &st = stage();
+  if (!st.active) return;
+  st.active = false;
+  if (st.upd.empty()) {
+    sync_shadows();
+    const int nr = (int)st.res.size();
+    if (st.rb == 8)
+      run_res_calc<double>(st.name.c_str(), st.eset, nr, st.res.data());
+    else
+      run_res_calc<float>(st.name.c_str(), st.eset, nr, st.res.data());
+    check_launch(st.name.c_str());
+    note_writes(nr, st.res.data());
+    if (!st.bres.empty()) {
+      flush_fold();
+      const int nb = (int)st.bres.size();
+      const op_set bset = st.bres[2].map->from;
+      if (st.rb == 8)
+        run_bres_calc<double>("bres_calc", bset, nb, st.bres.data());
+      else
+        run_bres_calc<float>("bres_calc", bset, nb, st.bres.data());
+      check_launch("bres_calc");
+      note_writes(nb, st.bres.data());
+    }
+    return;
+  }
+  if (st.rb == 8)
+    run_stage<double>(st, !next_takes_res);
+  else
+    run_stage<float>(st, !next_takes_res);
+}
 }  // namespace
 
 void fusion_flush() {
+  launch_stage(false);
   release_held(nullptr);
+  sync_shadows();
   flush_fold();
   zero_dats().clear();
 }
 void fusion_drop() {
+  stage().active = false;
+  for (auto &kv : in_shadow()) kv.second = false;
   held().active = false;
   held().save_args.clear();
   held().bres_args.clear();
@@ -2183,6 +2704,27 @@ int run_loop(const char *name, op_set set, int nargs, op_arg *args) {
 
   Runtime &r = rt();
   const bool fuse = r.capturing && r.graph_fusion;
+  if (fuse) {
+    if (stage_takes(l, rb, set, nargs, args)) return 0;
+    Plan *sp = starts_stage(name, l, rb, set, nargs, args);
+    launch_stage(sp && stage().res.size() > 6 && stage().res[6].dat == args[6].dat);
+    if (sp) {
+      // a stage reads q and adt where they are (the previous stage's
+      // outputs); anything held from before is launched first
+      release_held(nullptr, overwrites_held_res(name, l, set, nargs, args));
+      Stage &st = stage();
+      st = Stage{};
+      st.active = true;
+      st.rb = rb;
+      st.plan = sp;
+      st.name = name;
+      st.eset = set;
+      st.res.assign(args, args + nargs);
+      st.zero_in = known_zero(args[6].dat);
+      return 0;
+    }
+    sync_shadows();
+  }
   if (fuse && fuses_with_held(l, rb, set, args)) {
     Held &h = held();
     if (h.l == Loop::kUpdate || h.l == Loop::kUpdateBatch) {

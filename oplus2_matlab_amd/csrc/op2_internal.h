@@ -106,6 +106,7 @@ struct Plan {
   std::vector<uint16_t> ell;
   int max_positions = 0, max_owned = 0, max_incs = 0;
   bool owned_identity = false;  // owned[i] == i: tiles own consecutive targets
+  bool airfoil_res = false;     // run by the res_calc kernels (graph stage fusion)
 
   // device copies
   int *d_blk_off = nullptr, *d_perm = nullptr, *d_col_off = nullptr,
@@ -204,7 +205,7 @@ struct Runtime {
   // graph capture of loop sequences on the runtime stream; with
   // graph_fusion, adjacent loops that can share a pass are fused there
   bool capturing = false;
-  bool graph_fusion = true;
+  int graph_fusion = 2;  // 0 off, 1 loop fusion, 2 also one kernel per Runge-Kutta stage
   std::vector<hipGraphExec_t> graphs;
   std::vector<void *> retired;  // replaced scratch buffers that graphs may use
   std::map<std::string, OnlineTune> online;
@@ -228,6 +229,10 @@ void ensure_device();
 void *dev_alloc(size_t bytes);
 void dev_free(void *p);
 void release_exec_caches();
+// buffers the stage kernels of a capture need (op2_graph_begin; a capture
+// cannot allocate), and forgetting a plan's when the plan is dropped
+void prepare_capture();
+void forget_plan_caches(const Plan *p);
 // loop fusion during graph capture: launch, or forget, a held loop
 void fusion_flush();
 void fusion_drop();

@@ -682,6 +682,7 @@ void drop_plan(Plan *p) {
     if (r.plans[i].get() == p) {
       if (r.device_ready) (void)hipStreamSynchronize(r.stream);
       p->release(live_graphs() ? &r.retired : nullptr);
+      forget_plan_caches(p);
       if (r.last_plan == p) r.last_plan = nullptr;
       r.plans.erase(r.plans.begin() + i);
       return;

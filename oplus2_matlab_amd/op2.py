@@ -464,15 +464,23 @@ def graph_destroy(gid: int) -> None:
     _call("op2_graph_destroy", gid)
 
 
-def set_graph_fusion(on: bool) -> bool:
-    """Loop fusion during graph capture (default on): bres_calc, update, the
-    next save_soln and adt_calc run as one kernel, the rms fold rides in the
-    next res_calc, and a res that a captured update zeroed is taken as zero;
-    the results are the same bit for bit.  Returns the previous setting."""
-    prev = _call("op2_set_graph_fusion", int(on))
+FUSION_OFF, FUSION_LOOPS, FUSION_STAGES = 0, 1, 2
+
+
+def set_graph_fusion(level) -> int:
+    """Loop fusion during graph capture.  FUSION_LOOPS: bres_calc, update,
+    the next save_soln and adt_calc run as one kernel, the rms fold rides in
+    the next res_calc, and a res that a captured update zeroed is taken as
+    zero.  FUSION_STAGES (the default; True means it): also one kernel per
+    Runge-Kutta stage, res_calc included, on owner-gather plans that own
+    every cell.  q, adt and res keep the separate loops' bits; with stages the
+    rms sums are grouped per tile (the same to double rounding).  Returns the
+    previous level."""
+    lvl = FUSION_STAGES if level is True else FUSION_OFF if level is False else int(level)
+    prev = _call("op2_set_graph_fusion", lvl)
     if prev < 0:
         raise OP2Error("op2_set_graph_fusion: not during graph capture")
-    return bool(prev)
+    return prev
 
 
 def plan_summary(name: str, set_: Set, args, strategy: int = STRATEGY_COLOURED,

@@ -43,20 +43,31 @@ def graph_run(mesh, c, dtype, niter, fusion, renumber=False, graph_iters=1):
     return hist, q, adt, res
 
 
+def same_rms(h1, h0, level):
+    """Loop fusion keeps the rms bits; stage kernels group the rms sum per
+    tile, so it agrees to double rounding."""
+    if level == op2.FUSION_STAGES:
+        return bool(np.all(np.abs(h1 - h0) <= 1e-14 * np.abs(h0)))
+    return np.array_equal(h1, h0)
+
+
+@pytest.mark.parametrize("level", [op2.FUSION_LOOPS, op2.FUSION_STAGES])
 @pytest.mark.parametrize("dtype", ["double", "float"])
 @pytest.mark.parametrize("renumber,graph_iters", [(False, 1), (True, 1), (False, 3), (True, 4)])
-def test_fused_graph_equals_unfused(runtime, dtype, renumber, graph_iters):
+def test_fused_graph_equals_unfused(runtime, level, dtype, renumber, graph_iters):
     """graph_iters > 1 adds the update + next save_soln + adt_calc fusion;
-    29 replayed iterations leave a remainder for the one-iteration graph."""
-    note(f"fused vs unfused graph, 40x20 random, {dtype}, renumber={renumber}, "
+    29 replayed iterations leave a remainder for the one-iteration graph.
+    Stage kernels run on tiles owning consecutive cells (renumbered) or a
+    list of cells (as given)."""
+    note(f"fused (level {level}) vs unfused graph, 40x20 random, {dtype}, renumber={renumber}, "
          f"{graph_iters} iterations per graph, 30 iterations")
     mesh = M.o_mesh(40, 20, "random", seed=3)
     c = M.flow_constants().as_array()
-    h1, q1, a1, r1 = graph_run(mesh, c, dtype, 30, True, renumber, graph_iters)
+    h1, q1, a1, r1 = graph_run(mesh, c, dtype, 30, level, renumber, graph_iters)
     op2.op_exit()
     op2.op_init([], 0)
     h0, q0, a0, r0 = graph_run(mesh, c, dtype, 30, False, renumber, graph_iters)
-    assert np.array_equal(h1, h0)
+    assert same_rms(h1, h0, level)
     assert np.array_equal(q1, q0) and np.array_equal(a1, a0) and np.array_equal(r1, r0)
     if dtype == "double":
         st, oh = O.airfoil_seq(mesh, c, 30, "f64")
@@ -79,17 +90,25 @@ def test_fused_equals_unfused_other_strategies(runtime, strategy):
     assert np.array_equal(h1, h0) and np.array_equal(q1, q0) and np.array_equal(a1, a0)
 
 
+@pytest.mark.parametrize("level", [op2.FUSION_LOOPS, op2.FUSION_STAGES])
+@pytest.mark.parametrize("renumber", [False, True])
 @pytest.mark.parametrize("graph_iters", [1, 4])
-def test_fused_batch_graph_equals_unfused(runtime, graph_iters):
-    note(f"fused vs unfused batch graph, 6 meshes, 12 iterations, {graph_iters} per graph")
+def test_fused_batch_graph_equals_unfused(runtime, level, renumber, graph_iters):
+    """Renumbered, every tile lies in one mesh and the stage kernels fold
+    each mesh's tiles; as given, tiles may straddle meshes and the stages
+    fall back to the separate kernels."""
+    note(f"fused (level {level}) vs unfused batch graph, 6 meshes, renumber={renumber}, "
+         f"12 iterations, {graph_iters} per graph")
     batch = M.mesh_batch(nmesh=6, seed=4, cells_lo=300, cells_hi=3000, ordering="random")
-    op2.set_graph_fusion(True)
-    h1, q1 = AF.run_airfoil_batch(batch, 12, "double", graph=True, graph_iters=graph_iters)
+    op2.set_graph_fusion(level)
+    h1, q1 = AF.run_airfoil_batch(batch, 12, "double", graph=True, graph_iters=graph_iters,
+                                  renumber=renumber)
     op2.op_exit()
     op2.op_init([], 0)
     op2.set_graph_fusion(False)
-    h0, q0 = AF.run_airfoil_batch(batch, 12, "double", graph=True, graph_iters=graph_iters)
-    assert np.array_equal(h1, h0) and np.array_equal(q1, q0)
+    h0, q0 = AF.run_airfoil_batch(batch, 12, "double", graph=True, graph_iters=graph_iters,
+                                  renumber=renumber)
+    assert same_rms(h1, h0, level) and np.array_equal(q1, q0)
     for i in range(batch.nmesh):
         _, h = O.airfoil_seq(batch.meshes[i], batch.consts[i], 12, "f64")
         assert np.max(np.abs(h1[:, i] - h) / h) < 1e-12
@@ -198,12 +217,13 @@ def test_back_to_back_updates_fold_in_order(runtime):
     """Two updates captured with nothing between them: the first one's
     reduction (carried over to the next kernel when fusion is on) must be
     folded before the second overwrites the partials.  Same log and state
-    as the capture without fusion, bit for bit."""
+    as the capture without fusion, bit for bit with loop fusion, and to
+    double rounding of the last rms with a stage kernel."""
     note("graph of update, update, res_calc: carried folds vs separate folds")
     mesh = M.o_mesh(40, 20, "random", seed=12)
     c = M.flow_constants().as_array()
     out = {}
-    for fusion in (True, False):
+    for fusion in (op2.FUSION_STAGES, op2.FUSION_LOOPS, op2.FUSION_OFF):
         op2.op_init([], 0)
         for loop in ("res_calc", "bres_calc"):
             op2.set_loop_config(loop, op2.STRATEGY_OWNER_GATHER, 128, 0)
@@ -228,9 +248,15 @@ def test_back_to_back_updates_fold_in_order(runtime):
         op2.set_deferred_reductions(False)
         op2.op_exit()
     op2.op_init([], 0)
-    assert out[True][0].size == 9
-    for x, y in zip(out[True], out[False]):
+    off = out[op2.FUSION_OFF]
+    assert off[0].size == 9
+    for x, y in zip(out[op2.FUSION_LOOPS], off):
         assert np.array_equal(x, y)
+    # the last update joins res_calc as a stage (one per graph: its q and adt
+    # come back from the second buffers at the end of the graph)
+    st = out[op2.FUSION_STAGES]
+    assert same_rms(st[0], off[0], op2.FUSION_STAGES)
+    assert np.array_equal(st[1], off[1]) and np.array_equal(st[2], off[2])
 
 
 def test_fusion_switch_refused_during_capture(runtime):
