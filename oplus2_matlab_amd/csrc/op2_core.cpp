@@ -763,6 +763,9 @@ int op2_graph_end(void) {
   hipGraphExec_t ex = nullptr;
   OP2_HIP_CHECK(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
   OP2_HIP_CHECK(hipGraphDestroy(g));
+  // the graph's launch resources go to the device now, not at its first
+  // launch
+  OP2_HIP_CHECK(hipGraphUpload(ex, r.stream));
   r.graphs.push_back(ex);
   return (int)r.graphs.size() - 1;
   API_END(-1)

@@ -905,9 +905,15 @@ struct StageArgs {
   int nseg;
 };
 
-template <typename T, bool OWNID, bool BRES, bool SAVE, bool ADT>
-__global__ __launch_bounds__(NT) void k_stage(StageArgs<T> s, BresIdx<T> bi, AirfoilConsts<T> k,
-                                              CarriedFold fold, bool zero_res, bool store_res) {
+// With one owned cell per thread the register budget is that of 6 waves
+// per SIMD (80 VGPRs): measured faster than the compiler's own choice
+// (about 100 VGPRs, 4 waves) by 7% at 2.88M cells, and than 8 waves (which
+// spills) by 13%.  Two cells per thread keep 4 waves (no spills).
+template <typename T, int OWN, bool OWNID, bool BRES, bool SAVE, bool ADT>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(OWN == 1 ? 6 : 4, 8))) void
+k_stage(
+    StageArgs<T> s, BresIdx<T> bi, AirfoilConsts<T> k, CarriedFold fold, bool zero_res,
+    bool store_res) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ double red[NT / kWave];
   if (carries_fold(fold)) carried_fold(fold, red);
@@ -918,25 +924,24 @@ __global__ __launch_bounds__(NT) void k_stage(StageArgs<T> s, BresIdx<T> bi, Air
   const int o0 = s.owned_off[b], no = s.owned_off[b + 1] - o0;
   T *fs = reinterpret_cast<T *>(smem);  // [4][np]
   const int tid = threadIdx.x;
-  const bool own = tid < no;
-  const int c = own ? (OWNID ? o0 + tid : s.owned[o0 + tid]) : 0;
-  // the owned cell's records, issued before the flux phase so that their
-  // latency overlaps it
-  ushort4 e4 = make_ushort4(0xffff, 0xffff, 0xffff, 0xffff);
-  T rpre[4] = {T(0), T(0), T(0), T(0)}, qo[4], a = T(1);
-  int4 nd = make_int4(0, 0, 0, 0);
-  int bidx = -1;
-  if (own) {
-    const op2_v4us v = __builtin_nontemporal_load((const op2_v4us *)(s.ell + o0 + tid));
-    e4 = make_ushort4(v.x, v.y, v.z, v.w);
-    if (!zero_res) Vec<T>::load4(s.res + 4 * (size_t)c, rpre);
-    Vec<T>::load4(s.qold + 4 * (size_t)c, qo);
-    a = s.adt_in[c];
-    if (ADT) nd = s.pcell[c];
-    if (BRES) {
-      const uint2 w = bi.word[c >> 5];
-      const unsigned bit = 1u << (c & 31);
-      bidx = (w.x & bit) ? (int)(w.y + __popc(w.x & (bit - 1u))) : -1;
+  // the owned cells' increment records and (unless res is known zero) res,
+  // issued before the flux phase so that their latency overlaps it
+  bool own[OWN];
+  int c[OWN];
+  ushort4 e4[OWN];
+  T rpre[OWN][4];
+#pragma unroll
+  for (int j = 0; j < OWN; j++) {
+    const int i = tid + j * NT;
+    own[j] = i < no;
+    c[j] = own[j] ? (OWNID ? o0 + i : s.owned[o0 + i]) : 0;
+    e4[j] = make_ushort4(0xffff, 0xffff, 0xffff, 0xffff);
+#pragma unroll
+    for (int d = 0; d < 4; d++) rpre[j][d] = T(0);
+    if (own[j]) {
+      const op2_v4us v = __builtin_nontemporal_load((const op2_v4us *)(s.ell + o0 + i));
+      e4[j] = make_ushort4(v.x, v.y, v.z, v.w);
+      if (!zero_res) Vec<T>::load4(s.res + 4 * (size_t)c[j], rpre[j]);
     }
   }
   for (int p = tid; p < np; p += NT) {
@@ -966,53 +971,66 @@ __global__ __launch_bounds__(NT) void k_stage(StageArgs<T> s, BresIdx<T> bi, Air
 #pragma unroll
     for (int d = 0; d < 4; d++) fs[d * np + p] = f[d];
   }
-  // the nodes of the owned cell for its new adt: gathered while the other
-  // waves finish their fluxes
-  T xs[4][2];
-  if (ADT && own) {
-    Vec<T>::load2(s.xc + 2 * (size_t)nd.x, xs[0]);
-    Vec<T>::load2(s.xc + 2 * (size_t)nd.y, xs[1]);
-    Vec<T>::load2(s.xc + 2 * (size_t)nd.z, xs[2]);
-    Vec<T>::load2(s.xc + 2 * (size_t)nd.w, xs[3]);
+  // the owned cells' update inputs (after the flux phase: fewer registers
+  // held through it) and, for the new adt, the nodes of each cell, gathered
+  // while the other waves finish their fluxes
+  T qo[OWN][4], a[OWN], xs[OWN][4][2];
+  int bidx[OWN];
+#pragma unroll
+  for (int j = 0; j < OWN; j++) {
+    a[j] = T(1);
+    bidx[j] = -1;
+    if (!own[j]) continue;
+    Vec<T>::load4(s.qold + 4 * (size_t)c[j], qo[j]);
+    a[j] = s.adt_in[c[j]];
+    if (BRES) {
+      const uint2 w = bi.word[c[j] >> 5];
+      const unsigned bit = 1u << (c[j] & 31);
+      bidx[j] = (w.x & bit) ? (int)(w.y + __popc(w.x & (bit - 1u))) : -1;
+    }
+    if (ADT) cell_nodes(s.pcell, s.xc, c[j], xs[j]);
   }
   __syncthreads();
   double acc = 0.0;
-  if (own) {
+#pragma unroll
+  for (int j = 0; j < OWN; j++) {
+    if (!own[j]) continue;
+    const int cj = c[j];
     T sum[4] = {T(0), T(0), T(0), T(0)};
-    ell_sum(fs, np, e4, sum);
+    ell_sum(fs, np, e4[j], sum);
     T r[4];
 #pragma unroll
-    for (int d = 0; d < 4; d++) r[d] = rpre[d] + sum[d];
-    if (BRES && bidx >= 0) {
+    for (int d = 0; d < 4; d++) r[d] = rpre[j][d] + sum[d];
+    if (BRES && bidx[j] >= 0) {
       T qc[4];
-      Vec<T>::load4(s.q_in + 4 * (size_t)c, qc);
-      const int4 rec[2] = {bi.rec[2 * bidx], bi.rec[2 * bidx + 1]};
+      Vec<T>::load4(s.q_in + 4 * (size_t)cj, qc);
+      const int4 rec[2] = {bi.rec[2 * bidx[j]], bi.rec[2 * bidx[j] + 1]};
 #pragma unroll
-      for (int j = 0; j < 2; j++) {
-        if (rec[j].z < 0) break;
+      for (int m = 0; m < 2; m++) {
+        if (rec[m].z < 0) break;
         T x1[2], x2[2], f[4];
-        Vec<T>::load2(bi.x + 2 * (size_t)rec[j].x, x1);
-        Vec<T>::load2(bi.x + 2 * (size_t)rec[j].y, x2);
-        const T *qi = bi.qinf_dat ? bi.qinf_dat + 4 * (size_t)rec[j].w : k.qinf;
-        boundary_flux(x1, x2, qc, a, bi.bound[rec[j].z], qi, k, f);
+        Vec<T>::load2(bi.x + 2 * (size_t)rec[m].x, x1);
+        Vec<T>::load2(bi.x + 2 * (size_t)rec[m].y, x2);
+        const T *qi = bi.qinf_dat ? bi.qinf_dat + 4 * (size_t)rec[m].w : k.qinf;
+        boundary_flux(x1, x2, qc, a[j], bi.bound[rec[m].z], qi, k, f);
 #pragma unroll
         for (int d = 0; d < 4; d++) r[d] += f[d];
       }
     }
-    const double adti = 1.0 / (double)a;
+    const double adti = 1.0 / (double)a[j];
     T qn[4];
 #pragma unroll
     for (int d = 0; d < 4; d++) {
       const double del = adti * (double)r[d];
-      qn[d] = (T)((double)qo[d] - del);
+      qn[d] = (T)((double)qo[j][d] - del);
       acc += del * del;
     }
-    Vec<T>::store4(s.q_out + 4 * (size_t)c, qn);
-    if (SAVE) Vec<T>::store4(s.qold_out + 4 * (size_t)c, qn);
-    s.adt_out[c] = ADT ? adt_of(xs[0], xs[1], xs[2], xs[3], qn, k) : a;
+    Vec<T>::store4(s.q_out + 4 * (size_t)cj, qn);
+    if (SAVE) Vec<T>::store4(s.qold_out + 4 * (size_t)cj, qn);
+    s.adt_out[cj] = ADT ? adt_of(xs[j][0], xs[j][1], xs[j][2], xs[j][3], qn, k) : a[j];
     if (store_res) {
       const T z[4] = {T(0), T(0), T(0), T(0)};
-      Vec<T>::store4(s.res + 4 * (size_t)c, z);
+      Vec<T>::store4(s.res + 4 * (size_t)cj, z);
     }
   }
   acc = block_reduce<kRedSum, NT>(acc, red);
@@ -1919,7 +1937,7 @@ void prepare_capture() {
   for (auto &up : r.plans) {
     const Plan *p = up.get();
     if (!p->airfoil_res || p->strategy != OP2_STRATEGY_OWNER_GATHER || !p->d_ell ||
-        p->max_owned > NT)
+        p->max_owned > 2 * NT)
       continue;
     const op_set cells = p->groups[p->owner_group].map->to;
     if ((int)p->owned.size() != cells->size) continue;
@@ -2517,20 +2535,22 @@ bool stage_takes(Loop l, int rb, op_set set, int nargs, op_arg *a) {
   return false;
 }
 
-#define OP2_STG(O, B, S, A) k_stage<T, O, B, S, A>
+#define OP2_STG(W, O, B, S, A) k_stage<T, W, O, B, S, A>
+#define OP2_STG8(W)                                                                       \
+  OP2_STG(W, false, false, false, false), OP2_STG(W, false, false, false, true),           \
+      OP2_STG(W, false, false, true, false), OP2_STG(W, false, false, true, true),         \
+      OP2_STG(W, false, true, false, false), OP2_STG(W, false, true, false, true),         \
+      OP2_STG(W, false, true, true, false), OP2_STG(W, false, true, true, true),           \
+      OP2_STG(W, true, false, false, false), OP2_STG(W, true, false, false, true),         \
+      OP2_STG(W, true, false, true, false), OP2_STG(W, true, false, true, true),           \
+      OP2_STG(W, true, true, false, false), OP2_STG(W, true, true, false, true),           \
+      OP2_STG(W, true, true, true, false), OP2_STG(W, true, true, true, true)
 template <typename T>
-auto stage_kernel(bool oid, bool bres, bool save, bool adt) {
-  decltype(&OP2_STG(true, true, true, true)) table[16] = {
-      OP2_STG(false, false, false, false), OP2_STG(false, false, false, true),
-      OP2_STG(false, false, true, false),  OP2_STG(false, false, true, true),
-      OP2_STG(false, true, false, false),  OP2_STG(false, true, false, true),
-      OP2_STG(false, true, true, false),   OP2_STG(false, true, true, true),
-      OP2_STG(true, false, false, false),  OP2_STG(true, false, false, true),
-      OP2_STG(true, false, true, false),   OP2_STG(true, false, true, true),
-      OP2_STG(true, true, false, false),   OP2_STG(true, true, false, true),
-      OP2_STG(true, true, true, false),    OP2_STG(true, true, true, true)};
-  return table[oid * 8 + bres * 4 + save * 2 + adt];
+auto stage_kernel(int own, bool oid, bool bres, bool save, bool adt) {
+  decltype(&OP2_STG(1, true, true, true, true)) table[32] = {OP2_STG8(1), OP2_STG8(2)};
+  return table[(own > 1) * 16 + oid * 8 + bres * 4 + save * 2 + adt];
 }
+#undef OP2_STG8
 #undef OP2_STG
 
 template <typename T>
@@ -2565,7 +2585,8 @@ void run_stage(Stage &st, bool store_res) {
   const int *tiles = st.batch ? tiles_of_meshes(*sb, st.upd[4].map) : nullptr;
   a.nseg = st.batch ? st.upd[4].map->to->size : 0;
   BresIdx<T> bi = has_bres ? bres_idx<T>(st.bres.data(), (int)st.bres.size(), false) : BresIdx<T>{};
-  auto kern = stage_kernel<T>(p->owned_identity, has_bres, save, has_adt);
+  auto kern = stage_kernel<T>((p->max_owned + NT - 1) / NT, p->owned_identity, has_bres, save,
+                              has_adt);
   const size_t lds = std::max(sizeof(T) * 4 * (size_t)p->max_positions, (size_t)64);
   allow_lds(kern, lds);
   const CarriedFold fold = take_fold(p->nblocks);

@@ -317,3 +317,21 @@ def test_autotuning_refused_while_graphs_exist(runtime):
     op2.set_deferred_reductions(False)
     got = op2.autotune("res_calc", a.edges, a.res_calc_args(), 1)
     assert got["strategy"] in range(1, 6)
+
+
+@pytest.mark.parametrize("dtype", ["double", "float"])
+@pytest.mark.parametrize("renumber", [False, True])
+def test_stage_kernels_with_two_cells_per_thread(runtime, dtype, renumber):
+    """Owner-gather tiles of 512 cells: the stage kernel's threads own two
+    cells each; fused equals unfused (q, adt and res bit for bit)."""
+    note(f"stage kernels on 512-cell tiles, 60x30 random, {dtype}, renumber={renumber}")
+    for loop in ("res_calc", "bres_calc"):
+        op2.set_loop_config(loop, op2.STRATEGY_OWNER_GATHER, 512, 0)
+    mesh = M.o_mesh(60, 30, "random", seed=14)
+    c = M.flow_constants().as_array()
+    h1, q1, a1, r1 = graph_run(mesh, c, dtype, 11, op2.FUSION_STAGES, renumber, 4)
+    op2.op_exit()
+    op2.op_init([], 0)
+    h0, q0, a0, r0 = graph_run(mesh, c, dtype, 11, op2.FUSION_OFF, renumber, 4)
+    assert same_rms(h1, h0, op2.FUSION_STAGES)
+    assert np.array_equal(q1, q0) and np.array_equal(a1, a0) and np.array_equal(r1, r0)
