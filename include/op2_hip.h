@@ -132,6 +132,19 @@ void op2_exit(void);
  * Returns 0 on success. */
 int op2_par_loop(char const *name, op_set set, int nargs, op_arg *args);
 
+/* No counterpart in the reference: op2_par_loop for callers that cannot
+ * allocate device memory (code under graph capture by the caller, tight
+ * serving loops, harnesses that forbid hipMalloc).  Runs save_soln,
+ * adt_calc, and the airfoil and jac1 update loops (their OP_GBL reductions
+ * included) without any device allocation: the reduction partials live in
+ * a fixed device array of the library (at most 2 results of 2048 workgroup
+ * partials), one per GPU, shared by every call on that GPU (calls that
+ * overlap on one GPU race).  Needs op2_init first and, for adt_calc, its
+ * map on the device (run the loop once through op2_par_loop); other loops
+ * are refused with an error.  Not timed (op_timing_output) and not counted
+ * by checkpoints.  Same results as op2_par_loop bit for bit.  Returns 0. */
+int op2_par_loop_noalloc(char const *name, op_set set, int nargs, op_arg *args);
+
 /* ---------------------------------------------------------------------------
  * Runtime controls (no counterpart in the reference)
  */

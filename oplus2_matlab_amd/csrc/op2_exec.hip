@@ -1745,13 +1745,14 @@ auto update_kernel(bool adt, bool save, bool xcd, bool bres) {
 
 // update; fused with the bres_calc of bres_a (nb arguments) before it and
 // the save_soln of save_a and/or the adt_calc of adt_a after it (all on the
-// same cells and state) when given
+// same cells and state) when given.  use: reduction scratch of the caller
+// (result slot and kReduceBlocks partials) instead of the runtime's.
 template <typename T>
 const double *run_update(op_set set, op_arg *a, const op_arg *adt_a = nullptr,
                          const op_arg *save_a = nullptr, const op_arg *bres_a = nullptr,
-                         int nb = 0, bool keep_res = false) {
+                         int nb = 0, bool keep_res = false, const ReduceScratch *use = nullptr) {
   const int grid = grid_for(set->size, kReduceBlocks);
-  ReduceScratch s = reduce_scratch(1, grid);
+  ReduceScratch s = use ? *use : reduce_scratch(1, grid);
   const bool xcd = sizeof(T) == 4 || (long long)grid * NT >= set->size;
   auto kern = update_kernel<T>(adt_a, save_a, xcd, bres_a);
   hipLaunchKernelGGL(kern, dim3(grid), dim3(NT), 0, rt().stream,
@@ -2073,9 +2074,10 @@ void run_jac_res(const char *name, op_set set, int nargs, op_arg *a) {
   }
 }
 
+// use: as for run_update (two results of kReduceBlocks partials each)
 template <typename T>
-const double *run_jac_update(op_set set, op_arg *a) {
-  ReduceScratch s = reduce_scratch(2);
+const double *run_jac_update(op_set set, op_arg *a, const ReduceScratch *use = nullptr) {
+  ReduceScratch s = use ? *use : reduce_scratch(2);
   const int grid = grid_for(set->size, kReduceBlocks);
   const T alpha = (T)get_const("alpha").as_double(0);
   hipLaunchKernelGGL(k_jac_update<T>, dim3(grid), dim3(NT), 0, rt().stream,
@@ -2952,4 +2954,111 @@ int autotune(const char *name, op_set set, int nargs, op_arg *args, int reps, in
   return 0;
 }
 
+// ---- allocation-free loop entry point --------------------------------------
+//
+// op2_par_loop_noalloc runs a loop whose only device scratch is the
+// reduction partials (airfoil and jac1 update) or none (save_soln, adt_calc)
+// without allocating device memory: the partials live in `scratch`, a
+// file-scope device array every GPU holds once, whose address is taken with
+// hipGetSymbolAddress on the first call on each device (the name itself is
+// only a host-side handle).  Maximum: 2 reduction results of kReduceBlocks
+// workgroup partials each, plus a 256-byte result slot; the partials are
+// fully written by each call (nothing needs clearing).  Calls that overlap
+// on one GPU share the array and race.
+
+namespace {
+
+constexpr int kNoAllocSlots = 2;
+constexpr size_t kScratchDoubles = 256 / sizeof(double) + (size_t)kNoAllocSlots * kReduceBlocks;
+constexpr int kMaxDevices = 64;
+
+__device__ __align__(256) double scratch[kScratchDoubles];
+
+double *scratch_of_device[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device (looked up once).
+ReduceScratch noalloc_scratch() {
+  int dev = 0;
+  OP2_HIP_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) fail("op2_par_loop_noalloc: device %d out of range", dev);
+  if (!scratch_of_device[dev]) {
+    void *p = nullptr;
+    OP2_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    scratch_of_device[dev] = (double *)p;
+  }
+  double *base = scratch_of_device[dev];
+  return {base + 256 / sizeof(double), base};
+}
+
+}  // namespace
+
+int par_loop_noalloc(const char *name, op_set set, int nargs, op_arg *args) {
+  Runtime &r = rt();
+  if (!name || !set || nargs <= 0 || !args) fail("op2_par_loop_noalloc: invalid arguments");
+  if (!r.device_ready)
+    fail("op2_par_loop_noalloc: the runtime has no device yet (call op_init / op2_init)");
+  if (r.capturing) fail("op2_par_loop_noalloc: not during graph capture");
+  if (OP_diags > 0) {
+    int ninds = 0;
+    for (int m = 0; m < nargs; m++) op_arg_check(set, m, args[m], &ninds, name);
+  }
+  for (int m = 0; m < nargs; m++) {
+    if (args[m].argtype != OP_ARG_DAT) continue;
+    if (!args[m].dat || !args[m].dat->data_d)
+      fail("op2_par_loop_noalloc: arg %d of %s has no device data", m, name);
+    if (args[m].map && !args[m].map->map_d)
+      fail("op2_par_loop_noalloc: map %s of loop %s is not on the device yet (run the loop "
+           "once through op2_par_loop)", args[m].map->name, name);
+  }
+  if (r.loops.count(name))
+    fail("op2_par_loop_noalloc: loop %s runs on the generic path, which allocates", name);
+  const Loop l = identify(name, set, nargs, args);
+  if (l != Loop::kUpdate && l != Loop::kJacUpdate && l != Loop::kSaveSoln &&
+      l != Loop::kAdtCalc)
+    fail("op2_par_loop_noalloc: loop %s needs plans or buffers the runtime allocates (only "
+         "save_soln, adt_calc and the airfoil and jac1 update loops run without)", name);
+  const int rb = real_bytes(nargs, args);
+  if (rb != 4 && rb != 8)
+    fail("op2_par_loop_noalloc: loop \"%s\" needs all its real dats to be double or all float",
+         name);
+  ReduceScratch s = noalloc_scratch();
+  switch (l) {
+    case Loop::kSaveSoln:
+      if (rb == 8) run_save_soln<double>(set, args); else run_save_soln<float>(set, args);
+      check_launch(name);
+      return 0;
+    case Loop::kAdtCalc:
+      if (rb == 8) run_adt_calc<double>(set, args); else run_adt_calc<float>(set, args);
+      check_launch(name);
+      return 0;
+    case Loop::kUpdate: {
+      const double *d = rb == 8
+          ? run_update<double>(set, args, nullptr, nullptr, nullptr, 0, false, &s)
+          : run_update<float>(set, args, nullptr, nullptr, nullptr, 0, false, &s);
+      if (r.deferred) return 0;
+      gbl_inc(args[4], fetch_results(d, 1)[0]);
+      return 0;
+    }
+    default: {
+      const double *d =
+          rb == 8 ? run_jac_update<double>(set, args, &s) : run_jac_update<float>(set, args, &s);
+      if (r.deferred) return 0;
+      const double *h = fetch_results(d, 2);
+      gbl_inc(args[3], h[0]);
+      gbl_max(args[4], h[1]);
+      return 0;
+    }
+  }
+}
+
 }  // namespace op2
+
+using namespace op2;
+
+extern "C" int op2_par_loop_noalloc(char const *name, op_set set, int nargs, op_arg *args) {
+  try {
+    return par_loop_noalloc(name, set, nargs, args);
+  } catch (const op2::Error &) {
+    return -1;
+  }
+}

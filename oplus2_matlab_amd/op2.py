@@ -111,6 +111,7 @@ SIGNATURES = {
                           C.POINTER(C.c_int)]),
     "op2_exit": (None, []),
     "op2_par_loop": (C.c_int, [C.c_char_p, _P, C.c_int, C.POINTER(OpArg)]),
+    "op2_par_loop_noalloc": (C.c_int, [C.c_char_p, _P, C.c_int, C.POINTER(OpArg)]),
     "op2_set_error_mode": (C.c_int, [C.c_int]),
     "op2_last_error": (C.c_char_p, []),
     "op2_clear_error": (None, []),
@@ -301,6 +302,16 @@ def op_par_loop(kernel, name: str, set_: Set, *args: OpArg) -> None:
     del kernel
     arr = (OpArg * len(args))(*args)
     rc = _call("op2_par_loop", _b(name), set_._p, len(args), arr)
+    if rc != 0:
+        raise OP2Error(lib().op2_last_error().decode())
+
+
+def par_loop_noalloc(name: str, set_: Set, *args: OpArg) -> None:
+    """op2_par_loop_noalloc: the loop without any device allocation (only
+    save_soln, adt_calc and the airfoil / jac1 update loops; see
+    include/op2_hip.h)."""
+    arr = (OpArg * len(args))(*args)
+    rc = _call("op2_par_loop_noalloc", _b(name), set_._p, len(args), arr)
     if rc != 0:
         raise OP2Error(lib().op2_last_error().decode())
 
