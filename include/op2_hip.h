@@ -28,7 +28,9 @@ extern "C" {
  */
 
 /* common/op_lib_core.c:68-92: diagnostics level; argv OP_BLOCK_SIZE=,
- * OP_PART_SIZE=, OP_CACHE_LINE_SIZE= */
+ * OP_PART_SIZE=, OP_CACHE_LINE_SIZE=.  OP_block_size (threads per
+ * workgroup of every loop without a block size of its own) must be 128, 256
+ * (the default) or 512. */
 void op_init_core(int argc, char **argv, int diags);
 
 /* common/op_lib_core.c:94-119 (size must be > 0) */
@@ -179,21 +181,34 @@ void op2_clear_error(void);
 #define OP2_STRATEGY_OWNER_GATHER 5
 
 /* Per-loop tuning: strategy, part size (elements per block, 0 = default)
- * and threads per block (0 = default).  name NULL applies to every loop. */
+ * and threads per workgroup of the loop's kernels (0 = OP_block_size, whose
+ * default is 256; otherwise 128, 256 or 512, else an error) -- the
+ * per-loop OP_BLOCK_SIZE_<k> of the reference's generated stubs
+ * (airfoil/res_calc_kernel.cu:200-206).  name NULL applies to every loop.
+ * Kernels that fuse several loops in a captured graph use the block size of
+ * the loop that starts them (res_calc for a Runge-Kutta stage, update for
+ * update + save_soln + adt_calc). */
 int op2_set_loop_config(char const *name, int strategy, int part_size,
                         int block_size);
 
+/* The configuration a loop runs with now (after op2_set_loop_config and
+ * the autotuners): strategy (0 = the default), part size (0 = default) and
+ * block size (threads per workgroup, resolved).  Returns 0. */
+int op2_get_loop_config(char const *name, int *strategy, int *part_size, int *block_size);
+
 /* Autotuner: runs the loop (with these arguments, on the live data) under
- * each candidate strategy and part size, reps times each, keeps the fastest
- * as the loop's configuration and reports it.  Dats the loop writes are
- * saved before and restored after, so the state is unchanged.  Only
- * indirect loops without OP_GBL results (res_calc, bres_calc, jac1 res).
- * Returns 0 on success. */
+ * each candidate strategy and part size, then the best of them with each
+ * block size (128, 256, 512 threads), reps times each, keeps the fastest as
+ * the loop's configuration and reports it (the block size through
+ * op2_get_loop_config).  Dats the loop writes are saved before and restored
+ * after, so the state is unchanged.  Only indirect loops without OP_GBL
+ * results (res_calc, bres_calc, jac1 res).  Returns 0 on success. */
 int op2_autotune_loop(char const *name, op_set set, int nargs, op_arg *args, int reps,
                       int *best_strategy, int *best_part, double *best_us);
 
 /* Online autotuner: from now on, the calls of loop `name` run each
- * candidate strategy and part size in turn, one untimed call then `reps`
+ * candidate strategy, part size and (for res_calc's owner-gather tiles)
+ * block size in turn, one untimed call then `reps`
  * timed calls per candidate, and every call is a real execution of the loop
  * (so the candidates are timed inside the program's own loop sequence, with
  * its cache state).  After the last candidate the fastest becomes the loop's

@@ -42,7 +42,17 @@ inline int type_error(const bool *, const char *type) { return strcmp(type, "boo
 
 extern int OP_diags;
 
-inline void op_init(int argc, char **argv, int diags) { op2_init(argc, argv, diags); }
+// The program's own -DOP_BLOCK_SIZE / -DOP_PART_SIZE apply first and argv's
+// OP_BLOCK_SIZE= / OP_PART_SIZE= after them (common/op_lib_core.c:67-92).
+inline void op_init(int argc, char **argv, int diags) {
+#ifdef OP_BLOCK_SIZE
+  OP_block_size = OP_BLOCK_SIZE;
+#endif
+#ifdef OP_PART_SIZE
+  OP_part_size = OP_PART_SIZE;
+#endif
+  op2_init(argc, argv, diags);
+}
 
 inline op_dat op_decl_dat_char(op_set set, int dim, char const *type, int size,
                                char *data, char const *name) {

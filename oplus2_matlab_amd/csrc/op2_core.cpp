@@ -338,6 +338,10 @@ void op_init_core(int argc, char **argv, int diags) {
       printf("\n OP_cache_line_size  = %d \n", OP_cache_line_size);
     }
   }
+  // the kernels are instantiated for 2, 4 and 8 waves of 64 threads
+  if (OP_block_size != 128 && OP_block_size != 256 && OP_block_size != 512)
+    fail("op_init error -- OP_block_size %d is not supported (128, 256 or 512 threads)",
+         OP_block_size);
   // OP2_RENUMBER=1 (or auto, on): renumber without a call in the program
   const char *rn = getenv("OP2_RENUMBER");
   rt().auto_renumber =
@@ -626,8 +630,9 @@ int op2_set_loop_config(char const *name, int strategy, int part_size, int block
   API_BEGIN
   if (strategy < 0 || strategy > OP2_STRATEGY_OWNER_GATHER) fail("op2: unknown strategy %d", strategy);
   if (part_size < 0 || part_size > 65535) fail("op2: part size %d out of range", part_size);
-  if (block_size != 0 && (block_size < 64 || block_size > 1024 || block_size % 64))
-    fail("op2: block size %d must be a multiple of 64 in [64, 1024]", block_size);
+  if (block_size != 0 && block_size != 128 && block_size != 256 && block_size != 512)
+    fail("op2: block size %d is not supported (0 for the default, or 128, 256 or 512 "
+         "threads)", block_size);
   LoopConfig c{strategy, part_size, block_size};
   if (name)
     rt().loop_configs[name] = c;
@@ -635,6 +640,17 @@ int op2_set_loop_config(char const *name, int strategy, int part_size, int block
     rt().default_config = c;
     rt().loop_configs.clear();
   }
+  return 0;
+  API_END(-1)
+}
+
+int op2_get_loop_config(char const *name, int *strategy, int *part_size, int *block_size) {
+  API_BEGIN
+  if (!name) fail("op2_get_loop_config: invalid arguments");
+  const LoopConfig c = config_for(name);
+  if (strategy) *strategy = c.strategy;
+  if (part_size) *part_size = c.part_size;
+  if (block_size) *block_size = c.block_size ? c.block_size : OP_block_size;
   return 0;
   API_END(-1)
 }

@@ -13,15 +13,20 @@
 // precision form: float state and flux arithmetic, with update's
 // qold - res/adt and every global reduction carried in double.
 #include <algorithm>
-#include <set>
 #include <cstring>
+#include <set>
+#include <type_traits>
 
 #include "op2_internal.h"
 #include "op2_kernels.h"
 
 namespace op2 {
 
-constexpr int NT = 256;  // threads per workgroup (4 waves)
+// Threads per workgroup of the kernels whose size is fixed (the folds of
+// reduction partials and the copies of graph buffers).  Every other kernel
+// takes its workgroup size NT as a template parameter (128, 256 or 512: the
+// loop's block size), and the host code launching it passes the same NT.
+constexpr int kFixedBlock = 256;
 
 // ============================================================================
 // kernels
@@ -33,6 +38,7 @@ typedef unsigned op2_v4u __attribute__((ext_vector_type(4)));
 // save_soln: 16-byte copy.  The copy (qold) is next read by update, after
 // adt_calc and res_calc: non-temporal stores keep it from displacing their
 // data in the caches.
+template <int NT>
 __global__ __launch_bounds__(NT) void k_copy16(const uint4 *__restrict__ src,
                                                uint4 *__restrict__ dst, long long n) {
   long long i = (long long)blockIdx.x * NT + threadIdx.x;
@@ -72,7 +78,7 @@ __device__ __forceinline__ T cell_adt(const int4 *__restrict__ pcell, const T *_
 }
 
 // adt_calc: one cell per thread
-template <typename T>
+template <int NT, typename T>
 __global__ __launch_bounds__(NT) void k_adt_calc(const int4 *__restrict__ pcell,
                                                  const T *__restrict__ x,
                                                  const T *__restrict__ q, T *__restrict__ adt,
@@ -89,7 +95,7 @@ __global__ __launch_bounds__(NT) void k_adt_calc(const int4 *__restrict__ pcell,
 // save_soln followed by adt_calc on the same state, as one pass (the two
 // loops of a captured graph fused): q is read once, copied to qold (bit for
 // bit, non-temporal as in k_copy16) and used for the cell's adt.
-template <typename T>
+template <int NT, typename T>
 __global__ __launch_bounds__(NT) void k_save_adt(const int4 *__restrict__ pcell,
                                                  const T *__restrict__ x,
                                                  const T *__restrict__ q, T *__restrict__ qold,
@@ -132,6 +138,7 @@ __device__ __forceinline__ bool carries_fold(const CarriedFold &f) {
   return f.nseg > 0 ? (int)blockIdx.x < f.nseg : (f.n > 0 && blockIdx.x == 0);
 }
 
+template <int NT>
 __device__ __forceinline__ void carried_fold(const CarriedFold &f, double *lds) {
   double acc = 0.0;
   if (f.nseg > 0) {
@@ -161,7 +168,7 @@ __device__ __forceinline__ void carried_fold(const CarriedFold &f, double *lds) 
 //     otherwise - LDS atomic adds;
 //  4. add the accumulator into res: plain read-modify-write when blocks of
 //     one launch share no cell (block colours), global atomics otherwise.
-template <typename T, int EPT, bool COLOURED>
+template <int NT, typename T, int EPT, bool COLOURED>
 __global__ __launch_bounds__(NT) void k_res_calc(
     const int *__restrict__ blk_off, const int *__restrict__ blkmap, int blk_first, int nblk,
     const int *__restrict__ node_off, const int *__restrict__ node_list,
@@ -171,7 +178,7 @@ __global__ __launch_bounds__(NT) void k_res_calc(
     const T *__restrict__ q, const T *__restrict__ adt, T *__restrict__ res,
     AirfoilConsts<T> k, CarriedFold fold) {
   extern __shared__ __align__(16) unsigned char smem[];
-  if (carries_fold(fold)) carried_fold(fold, reinterpret_cast<double *>(smem));
+  if (carries_fold(fold)) carried_fold<NT>(fold, reinterpret_cast<double *>(smem));
   const int b = blkmap[blk_first + xcd_block(blockIdx.x, nblk)];
   const int e0 = blk_off[b], ne = blk_off[b + 1] - e0;
   const int n0 = node_off[b], nn = node_off[b + 1] - n0;
@@ -265,7 +272,7 @@ __global__ __launch_bounds__(NT) void k_res_calc(
 //  3. one thread per owned cell sums its increments in the plan's fixed
 //     order and adds them to res with a plain read-modify-write.
 // No atomics, no colours, one launch, deterministic.
-template <typename T, int OWN, bool ELL>
+template <int NT, typename T, int OWN, bool ELL>
 __global__ __launch_bounds__(NT) void k_res_calc_owner(
     const int *__restrict__ blk_off, int nblk, const int *__restrict__ node_off,
     const int *__restrict__ node_list, const int *__restrict__ cell_off,
@@ -276,7 +283,7 @@ __global__ __launch_bounds__(NT) void k_res_calc_owner(
     const T *__restrict__ q, const T *__restrict__ adt, T *__restrict__ res,
     AirfoilConsts<T> k, CarriedFold fold, bool zero_res) {
   extern __shared__ __align__(16) unsigned char smem[];
-  if (carries_fold(fold)) carried_fold(fold, reinterpret_cast<double *>(smem));
+  if (carries_fold(fold)) carried_fold<NT>(fold, reinterpret_cast<double *>(smem));
   constexpr int pad = 16 / sizeof(T);
   const int b = xcd_block(blockIdx.x, nblk);
   const int p0 = blk_off[b], np = blk_off[b + 1] - p0;
@@ -391,7 +398,7 @@ __device__ __forceinline__ void ell_sum(const T *fs, int np, ushort4 e4, T acc[4
 typedef int op2_v4i __attribute__((ext_vector_type(4)));
 typedef unsigned short op2_v4us __attribute__((ext_vector_type(4)));
 
-template <typename T, int OWN, bool ELL, bool OWNID>
+template <int NT, typename T, int OWN, bool ELL, bool OWNID>
 __global__ __launch_bounds__(NT) void k_res_calc_owner_direct(
     const int *__restrict__ blk_off, int nblk, const int4 *__restrict__ ids,
     const int *__restrict__ owned_off, const int *__restrict__ owned,
@@ -400,7 +407,7 @@ __global__ __launch_bounds__(NT) void k_res_calc_owner_direct(
     const T *__restrict__ adt, T *__restrict__ res, AirfoilConsts<T> k, CarriedFold fold,
     bool zero_res) {
   extern __shared__ __align__(16) unsigned char smem[];
-  if (carries_fold(fold)) carried_fold(fold, reinterpret_cast<double *>(smem));
+  if (carries_fold(fold)) carried_fold<NT>(fold, reinterpret_cast<double *>(smem));
   const int b = xcd_block(blockIdx.x, nblk);
   const int p0 = blk_off[b], np = blk_off[b + 1] - p0;
   const int o0 = owned_off[b], no = owned_off[b + 1] - o0;
@@ -494,7 +501,7 @@ __global__ __launch_bounds__(NT) void k_res_calc_owner_direct(
 }
 
 // Owner-compute boundary loop (bres_calc): one increment per element.
-template <typename T>
+template <int NT, typename T>
 __global__ __launch_bounds__(NT) void k_bres_calc_owner(
     const int *__restrict__ blk_off, int nblk, const int *__restrict__ perm,
     const int *__restrict__ node_off, const int *__restrict__ node_list,
@@ -563,7 +570,7 @@ __global__ __launch_bounds__(NT) void k_bres_calc_owner(
 // gathers x, q and adt straight from global memory and adds its flux to
 // both cells with global atomics.  No barriers and no LDS, so every wave
 // keeps its loads in flight; the L2 supplies the reuse between edges.
-template <typename T>
+template <int NT, typename T>
 __global__ __launch_bounds__(NT) void k_res_calc_direct(const int2 *__restrict__ pedge,
                                                         const int2 *__restrict__ pecell,
                                                         const T *__restrict__ x,
@@ -573,7 +580,7 @@ __global__ __launch_bounds__(NT) void k_res_calc_direct(const int2 *__restrict__
                                                         AirfoilConsts<T> k, CarriedFold fold) {
   if (carries_fold(fold)) {
     __shared__ double lds[NT / kWave];
-    carried_fold(fold, lds);
+    carried_fold<NT>(fold, lds);
   }
   const int e = xcd_block(blockIdx.x, gridDim.x) * NT + threadIdx.x;
   if (e >= nedge) return;
@@ -596,7 +603,7 @@ __global__ __launch_bounds__(NT) void k_res_calc_direct(const int2 *__restrict__
 // edge, global atomics.  The loop is small (one edge per boundary cell on
 // the O-mesh), so its time is latency: a single pass of dependent loads.
 // qinf_dat / bmap: the batched form's free stream of the edge's mesh.
-template <typename T>
+template <int NT, typename T>
 __global__ __launch_bounds__(NT) void k_bres_calc_direct(
     const int2 *__restrict__ pbedge, const int *__restrict__ pbecell, const T *__restrict__ x,
     const T *__restrict__ q, const T *__restrict__ adt, T *__restrict__ res,
@@ -618,7 +625,7 @@ __global__ __launch_bounds__(NT) void k_bres_calc_direct(
 
 // Staged boundary-edge loop (bres_calc): as k_res_calc with one increment per
 // element and the direct `bound` read through the plan's element order.
-template <typename T, bool COLOURED>
+template <int NT, typename T, bool COLOURED>
 __global__ __launch_bounds__(NT) void k_bres_calc(
     const int *__restrict__ blk_off, const int *__restrict__ blkmap, int blk_first, int nblk,
     const int *__restrict__ perm, const int *__restrict__ node_off,
@@ -783,7 +790,7 @@ __device__ __forceinline__ void upd_store(int c, const UpdIn<T, ADT> &in, T *q, 
 // set (measured: 180k cells, +27% cell-updates/s) and for float state
 // (2.88M cells, FP32 form +2.5%).  Large FP64 sets keep the plain
 // interleaved order, which streams faster there (2.88M cells: 1.6-4%).
-template <typename T, bool ADT, bool SAVE, bool XCD, bool BRES>
+template <int NT, typename T, bool ADT, bool SAVE, bool XCD, bool BRES>
 __global__ __launch_bounds__(NT) void k_update(const T *qold, T *q, T *__restrict__ res,
                                                const T *adt, int ncell,
                                                double *__restrict__ partials,
@@ -824,7 +831,7 @@ __global__ __launch_bounds__(NT) void k_update(const T *qold, T *q, T *__restric
 // its range meets, writes one partial (block tree over its threads, in a
 // fixed order) at slot pslot[m] + b - (first workgroup of mesh m).
 // k_fold_seg adds a mesh's partials in workgroup order.
-template <typename T, int UPC, bool ADT, bool SAVE, bool BRES>
+template <int NT, typename T, int UPC, bool ADT, bool SAVE, bool BRES>
 __global__ __launch_bounds__(NT) void k_update_seg(const T *qold, T *q,
                                                    T *__restrict__ res, const T *adt,
                                                    int ncell, const int *__restrict__ cmap,
@@ -909,14 +916,14 @@ struct StageArgs {
 // per SIMD (80 VGPRs): measured faster than the compiler's own choice
 // (about 100 VGPRs, 4 waves) by 7% at 2.88M cells, and than 8 waves (which
 // spills) by 13%.  Two cells per thread keep 4 waves (no spills).
-template <typename T, int OWN, bool OWNID, bool BRES, bool SAVE, bool ADT>
+template <int NT, typename T, int OWN, bool OWNID, bool BRES, bool SAVE, bool ADT>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(OWN == 1 ? 6 : 4, 8))) void
 k_stage(
     StageArgs<T> s, BresIdx<T> bi, AirfoilConsts<T> k, CarriedFold fold, bool zero_res,
     bool store_res) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ double red[NT / kWave];
-  if (carries_fold(fold)) carried_fold(fold, red);
+  if (carries_fold(fold)) carried_fold<NT>(fold, red);
   if (s.log_count && blockIdx.x == 0 && threadIdx.x == 0)
     *s.log_base = atomicAdd(s.log_count, (unsigned)s.nseg);
   const int b = xcd_block(blockIdx.x, s.nblk);
@@ -1041,11 +1048,12 @@ k_stage(
 // the block tree: a fixed order); the dat gets dat[m] += out and, with
 // deferred reductions, the log slot reserved by k_update_seg gets out (the
 // results of one call in mesh order).
-__global__ __launch_bounds__(NT) void k_fold_seg(const double *__restrict__ partials,
+__global__ __launch_bounds__(kFixedBlock) void k_fold_seg(const double *__restrict__ partials,
                                                  const int *__restrict__ pslot,
                                                  double *__restrict__ dat,
                                                  double *__restrict__ log,
                                                  const unsigned *__restrict__ log_base) {
+  constexpr int NT = kFixedBlock;
   __shared__ double lds[NT / kWave];
   const int m = blockIdx.x;
   double acc = 0.0;
@@ -1063,10 +1071,11 @@ __global__ __launch_bounds__(NT) void k_fold_seg(const double *__restrict__ part
 // One workgroup; the kernel boundary after the producer makes the partials
 // visible without device-scope fences.
 template <int NSLOT>
-__global__ __launch_bounds__(NT) void k_fold(const double *__restrict__ partials, int nparts,
+__global__ __launch_bounds__(kFixedBlock) void k_fold(const double *__restrict__ partials, int nparts,
                                              unsigned maxmask, double *__restrict__ out,
                                              double *__restrict__ log,
                                              unsigned *__restrict__ log_count) {
+  constexpr int NT = kFixedBlock;
   __shared__ double lds[NT / kWave];
 #pragma unroll
   for (int s = 0; s < NSLOT; s++) {
@@ -1090,7 +1099,7 @@ __global__ __launch_bounds__(NT) void k_fold(const double *__restrict__ partials
 
 // Staged Jacobi edge loop (jac1 res): du[pp0] += beta * A * u[pp1]; one map,
 // slots (pp, 1) for u and (pp, 0) for du.
-template <typename T, bool COLOURED>
+template <int NT, typename T, bool COLOURED>
 __global__ __launch_bounds__(NT) void k_jac_res(
     const int *__restrict__ blk_off, const int *__restrict__ blkmap, int blk_first, int nblk,
     const int *__restrict__ perm, const int *__restrict__ node_off,
@@ -1144,7 +1153,7 @@ __global__ __launch_bounds__(NT) void k_jac_res(
 
 // Owner-compute Jacobi edge loop (jac1 res): nodes cut into runs, each
 // workgroup sums the contributions to its nodes in a fixed order.
-template <typename T>
+template <int NT, typename T>
 __global__ __launch_bounds__(NT) void k_jac_res_owner(
     const int *__restrict__ blk_off, int nblk, const int *__restrict__ perm,
     const int *__restrict__ node_off, const int *__restrict__ node_list,
@@ -1174,7 +1183,7 @@ __global__ __launch_bounds__(NT) void k_jac_res_owner(
 
 // jac1 update: u += du + alpha r, du = 0, u_sum += u^2, u_max = max(u_max, u)
 // (partials: u_sum of each workgroup, then u_max of each workgroup)
-template <typename T>
+template <int NT, typename T>
 __global__ __launch_bounds__(NT) void k_jac_update(const T *__restrict__ r, T *__restrict__ du,
                                                    T *__restrict__ u, int n, T alpha,
                                                    double *__restrict__ partials) {
@@ -1285,8 +1294,27 @@ AirfoilConsts<T> airfoil_consts() {
   return k;
 }
 
-int grid_for(long long n, int cap) {
-  long long g = (n + NT - 1) / NT;
+// Threads per workgroup of a loop's kernels: the loop's configured block
+// size, else OP_block_size (OP_BLOCK_SIZE at op_init; 256 by default).
+int block_of(const std::string &loop) {
+  const int b = config_for(loop).block_size;
+  return b ? b : OP_block_size;
+}
+
+// Calls f(std::integral_constant<int, B>()) with B = bs, the block size the
+// kernels are instantiated for (128, 256 or 512).
+template <class F>
+auto with_block(int bs, F &&f) {
+  switch (bs) {
+    case 128: return f(std::integral_constant<int, 128>());
+    case 512: return f(std::integral_constant<int, 512>());
+    case 256: return f(std::integral_constant<int, 256>());
+  }
+  fail("op2: block size %d is not supported (128, 256 or 512 threads)", bs);
+}
+
+int grid_for(long long n, int cap, int bs = kFixedBlock) {
+  long long g = (n + bs - 1) / bs;
   if (g < 1) g = 1;
   return (int)std::min<long long>(g, cap);
 }
@@ -1358,26 +1386,26 @@ double gbl_read(const op_arg &a) {
 
 // ---- per-loop launchers ------------------------------------------------------
 
-template <typename T>
+template <int NT, typename T>
 void run_save_soln(op_set set, op_arg *a) {
   const long long chunks = (long long)set->size * 4 * sizeof(T) / 16;
-  hipLaunchKernelGGL(k_copy16, dim3(grid_for(chunks, 1 << 20)), dim3(NT), 0, rt().stream,
+  hipLaunchKernelGGL((k_copy16<NT>), dim3(grid_for(chunks, 1 << 20, NT)), dim3(NT), 0, rt().stream,
                      (const uint4 *)a[0].dat->data_d, (uint4 *)a[1].dat->data_d, chunks);
 }
 
 // save_soln (a) followed by adt_calc (adt_a) on the q it copies
-template <typename T>
+template <int NT, typename T>
 void run_save_adt(op_set set, const op_arg *a, const op_arg *adt_a) {
-  hipLaunchKernelGGL(k_save_adt<T>, dim3(grid_for(set->size, 1 << 30)), dim3(NT), 0,
+  hipLaunchKernelGGL((k_save_adt<NT, T>), dim3(grid_for(set->size, 1 << 30, NT)), dim3(NT), 0,
                      rt().stream, (const int4 *)adt_a[0].map->map_d,
                      (const T *)adt_a[0].dat->data_d, (const T *)a[0].dat->data_d,
                      (T *)a[1].dat->data_d, (T *)adt_a[5].dat->data_d, set->size,
                      airfoil_consts<T>());
 }
 
-template <typename T>
+template <int NT, typename T>
 void run_adt_calc(op_set set, op_arg *a) {
-  hipLaunchKernelGGL(k_adt_calc<T>, dim3(grid_for(set->size, 1 << 30)), dim3(NT), 0,
+  hipLaunchKernelGGL((k_adt_calc<NT, T>), dim3(grid_for(set->size, 1 << 30, NT)), dim3(NT), 0,
                      rt().stream, (const int4 *)a[0].map->map_d, (const T *)a[0].dat->data_d,
                      (const T *)a[4].dat->data_d, (T *)a[5].dat->data_d, set->size,
                      airfoil_consts<T>());
@@ -1423,10 +1451,10 @@ void flush_fold() {
   if (!p.active) return;
   p.active = false;
   if (p.f.nseg > 0)
-    hipLaunchKernelGGL(k_fold_seg, dim3(p.f.nseg), dim3(NT), 0, rt().stream, p.f.partials,
+    hipLaunchKernelGGL(k_fold_seg, dim3(p.f.nseg), dim3(kFixedBlock), 0, rt().stream, p.f.partials,
                        p.f.pslot, p.f.dat, p.f.log, p.f.log_base);
   else
-    hipLaunchKernelGGL(k_fold<1>, dim3(1), dim3(NT), 0, rt().stream, p.f.partials, p.f.n, 0u,
+    hipLaunchKernelGGL((k_fold<1>), dim3(1), dim3(kFixedBlock), 0, rt().stream, p.f.partials, p.f.n, 0u,
                        p.f.out, p.f.log, p.f.log_count);
   check_launch("update");
 }
@@ -1441,12 +1469,15 @@ CarriedFold take_fold(int nblocks) {
   return p.f;
 }
 
-template <typename T, int EPT>
+template <int NT, typename T, int EPT>
 void launch_res_calc(Plan *p, op_arg *a, bool coloured) {
   const PlanGroup &nodes = p->groups[p->slots[0].first];
   const PlanGroup &cells = p->groups[p->slots[2].first];
-  const size_t lds = sizeof(T) * (2 * ((nodes.max_count + 1) & ~1) + 5 * (size_t)cells.max_count);
-  auto kern = coloured ? k_res_calc<T, EPT, true> : k_res_calc<T, EPT, false>;
+  // (at least the carried fold's NT / 64 doubles)
+  const size_t lds = std::max(
+      sizeof(T) * (2 * ((nodes.max_count + 1) & ~1) + 5 * (size_t)cells.max_count),
+      sizeof(double) * (NT / kWave));
+  auto kern = coloured ? k_res_calc<NT, T, EPT, true> : k_res_calc<NT, T, EPT, false>;
   if (lds > 64 * 1024)
     OP2_HIP_CHECK(hipFuncSetAttribute((const void *)kern,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
@@ -1489,7 +1520,7 @@ void allow_lds(K kern, size_t lds) {
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
 }
 
-template <typename T>
+template <int NT, typename T>
 void run_res_calc(const char *name, op_set set, int nargs, op_arg *a) {
   const int strategy = strategy_of(name);
   if (strategy == OP2_STRATEGY_OWNER) {
@@ -1500,12 +1531,12 @@ void run_res_calc(const char *name, op_set set, int nargs, op_arg *a) {
     if (own > 4) fail("op2: owner tile of %s exceeds %d targets", name, 4 * NT);
     const size_t lds = owner_lds<T>(p, 2, 4, 4);
     const bool ell = p->d_ell != nullptr;
-    auto kern = ell ? (own <= 1 ? k_res_calc_owner<T, 1, true>
-                                : own <= 2 ? k_res_calc_owner<T, 2, true>
-                                           : k_res_calc_owner<T, 4, true>)
-                    : (own <= 1 ? k_res_calc_owner<T, 1, false>
-                                : own <= 2 ? k_res_calc_owner<T, 2, false>
-                                           : k_res_calc_owner<T, 4, false>);
+    auto kern = ell ? (own <= 1 ? k_res_calc_owner<NT, T, 1, true>
+                                : own <= 2 ? k_res_calc_owner<NT, T, 2, true>
+                                           : k_res_calc_owner<NT, T, 4, true>)
+                    : (own <= 1 ? k_res_calc_owner<NT, T, 1, false>
+                                : own <= 2 ? k_res_calc_owner<NT, T, 2, false>
+                                           : k_res_calc_owner<NT, T, 4, false>);
     allow_lds(kern, std::max(lds, (size_t)64));
     if (p->nblocks == 0) {
       flush_fold();
@@ -1528,9 +1559,9 @@ void run_res_calc(const char *name, op_set set, int nargs, op_arg *a) {
     if (own > 4) fail("op2: owner tile of %s exceeds %d targets", name, 4 * NT);
     const bool ell = p->d_ell != nullptr;
     const bool oid = p->owned_identity;
-    decltype(&k_res_calc_owner_direct<T, 1, true, true>) kern;
+    decltype(&k_res_calc_owner_direct<NT, T, 1, true, true>) kern;
 #define OP2_G(O, E) \
-  (oid ? k_res_calc_owner_direct<T, O, E, true> : k_res_calc_owner_direct<T, O, E, false>)
+  (oid ? k_res_calc_owner_direct<NT, T, O, E, true> : k_res_calc_owner_direct<NT, T, O, E, false>)
     if (ell)
       kern = own <= 1 ? OP2_G(1, true) : own <= 2 ? OP2_G(2, true) : OP2_G(4, true);
     else
@@ -1551,8 +1582,8 @@ void run_res_calc(const char *name, op_set set, int nargs, op_arg *a) {
     return;
   }
   if (strategy == OP2_STRATEGY_DIRECT) {
-    const int grid = grid_for(set->size, 1 << 30);
-    hipLaunchKernelGGL(k_res_calc_direct<T>, dim3(grid), dim3(NT), 0, rt().stream,
+    const int grid = grid_for(set->size, 1 << 30, NT);
+    hipLaunchKernelGGL((k_res_calc_direct<NT, T>), dim3(grid), dim3(NT), 0, rt().stream,
                        (const int2 *)a[0].map->map_d, (const int2 *)a[2].map->map_d,
                        (const T *)a[0].dat->data_d, (const T *)a[2].dat->data_d,
                        (const T *)a[4].dat->data_d, (T *)a[6].dat->data_d, set->size,
@@ -1564,11 +1595,11 @@ void run_res_calc(const char *name, op_set set, int nargs, op_arg *a) {
   Plan *p = plan_for(name, set, nargs, a, strategy, part);
   const bool col = strategy == OP2_STRATEGY_COLOURED;
   if (part <= NT)
-    launch_res_calc<T, 1>(p, a, col);
+    launch_res_calc<NT, T, 1>(p, a, col);
   else if (part <= 2 * NT)
-    launch_res_calc<T, 2>(p, a, col);
+    launch_res_calc<NT, T, 2>(p, a, col);
   else
-    launch_res_calc<T, 4>(p, a, col);
+    launch_res_calc<NT, T, 4>(p, a, col);
 }
 
 // loops with fewer kernel forms than res_calc map the other strategies on
@@ -1658,7 +1689,7 @@ BresIdx<T> bres_idx(const op_arg *b, int nb, bool keep_res) {
   return bi;
 }
 
-template <typename T>
+template <int NT, typename T>
 void run_bres_calc(const char *name, op_set set, int nargs, op_arg *a) {
   if (rt().graph_fusion) build_bres_index(nargs, a);  // for captures that fuse it
   const int requested = strategy_of(name);
@@ -1668,7 +1699,7 @@ void run_bres_calc(const char *name, op_set set, int nargs, op_arg *a) {
   const int *bm = nargs == 7 ? a[6].map->map_d : nullptr;
   if (requested == OP2_STRATEGY_DIRECT) {
     if (set->size == 0) return;
-    hipLaunchKernelGGL(k_bres_calc_direct<T>, dim3((set->size + NT - 1) / NT), dim3(NT), 0,
+    hipLaunchKernelGGL((k_bres_calc_direct<NT, T>), dim3((set->size + NT - 1) / NT), dim3(NT), 0,
                        rt().stream, (const int2 *)a[0].map->map_d, (const int *)a[2].map->map_d,
                        (const T *)a[0].dat->data_d, (const T *)a[2].dat->data_d,
                        (const T *)a[3].dat->data_d, (T *)a[4].dat->data_d,
@@ -1680,9 +1711,9 @@ void run_bres_calc(const char *name, op_set set, int nargs, op_arg *a) {
     const PlanGroup &nodes = p->groups[p->slots[0].first];
     const PlanGroup &cells = p->groups[p->owner_group];
     const size_t lds = owner_lds<T>(p, 2, 4, 4);
-    allow_lds(k_bres_calc_owner<T>, lds);
+    allow_lds(k_bres_calc_owner<NT, T>, lds);
     if (p->nblocks == 0) return;
-    hipLaunchKernelGGL(k_bres_calc_owner<T>, dim3(p->nblocks), dim3(NT), lds, rt().stream,
+    hipLaunchKernelGGL((k_bres_calc_owner<NT, T>), dim3(p->nblocks), dim3(NT), lds, rt().stream,
                        p->d_blk_off, p->nblocks, p->d_perm, nodes.d_off, nodes.d_list,
                        cells.d_off, cells.d_list, (const ushort4 *)p->d_loc, p->d_owned_off,
                        p->d_owned, p->d_csr_off, p->d_csr, (const T *)a[0].dat->data_d,
@@ -1697,7 +1728,7 @@ void run_bres_calc(const char *name, op_set set, int nargs, op_arg *a) {
   const PlanGroup &cells = p->groups[p->slots[2].first];
   const size_t lds = sizeof(T) * (2 * ((nodes.max_count + 1) & ~1) + 4 * (size_t)cells.max_count +
                                   ((cells.max_count + 3) & ~3) + 4 * (size_t)cells.max_count);
-  auto kern = col ? k_bres_calc<T, true> : k_bres_calc<T, false>;
+  auto kern = col ? k_bres_calc<NT, T, true> : k_bres_calc<NT, T, false>;
   if (lds > 64 * 1024)
     OP2_HIP_CHECK(hipFuncSetAttribute((const void *)kern,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
@@ -1727,8 +1758,8 @@ ReduceScratch reduce_scratch(int nslots, int nparts = kReduceBlocks) {
   return {(double *)(base + 256), (double *)base};
 }
 
-#define OP2_UPD(A, S, X, B) k_update<T, A, S, X, B>
-template <typename T>
+#define OP2_UPD(A, S, X, B) k_update<NT, T, A, S, X, B>
+template <int NT, typename T>
 auto update_kernel(bool adt, bool save, bool xcd, bool bres) {
   decltype(&OP2_UPD(false, false, false, false)) table[16] = {
       OP2_UPD(false, false, false, false), OP2_UPD(false, false, false, true),
@@ -1747,14 +1778,14 @@ auto update_kernel(bool adt, bool save, bool xcd, bool bres) {
 // the save_soln of save_a and/or the adt_calc of adt_a after it (all on the
 // same cells and state) when given.  use: reduction scratch of the caller
 // (result slot and kReduceBlocks partials) instead of the runtime's.
-template <typename T>
+template <int NT, typename T>
 const double *run_update(op_set set, op_arg *a, const op_arg *adt_a = nullptr,
                          const op_arg *save_a = nullptr, const op_arg *bres_a = nullptr,
                          int nb = 0, bool keep_res = false, const ReduceScratch *use = nullptr) {
-  const int grid = grid_for(set->size, kReduceBlocks);
+  const int grid = grid_for(set->size, kReduceBlocks, NT);
   ReduceScratch s = use ? *use : reduce_scratch(1, grid);
   const bool xcd = sizeof(T) == 4 || (long long)grid * NT >= set->size;
-  auto kern = update_kernel<T>(adt_a, save_a, xcd, bres_a);
+  auto kern = update_kernel<NT, T>(adt_a, save_a, xcd, bres_a);
   hipLaunchKernelGGL(kern, dim3(grid), dim3(NT), 0, rt().stream,
                      (const T *)a[0].dat->data_d, (T *)a[1].dat->data_d, (T *)a[2].dat->data_d,
                      (const T *)a[3].dat->data_d, set->size, s.partials,
@@ -1774,7 +1805,7 @@ const double *run_update(op_set set, op_arg *a, const op_arg *adt_a = nullptr,
   if (r.capturing && r.graph_fusion) {
     pending_fold() = PendingFold{true, f};  // the next res_calc carries it
   } else {
-    hipLaunchKernelGGL(k_fold<1>, dim3(1), dim3(NT), 0, r.stream, f.partials, f.n, 0u, f.out,
+    hipLaunchKernelGGL((k_fold<1>), dim3(1), dim3(kFixedBlock), 0, r.stream, f.partials, f.n, 0u, f.out,
                        f.log, f.log_count);
   }
   check_launch("update");
@@ -1785,6 +1816,7 @@ const double *run_update(op_set set, op_arg *a, const op_arg *adt_a = nullptr,
 // update: run offsets and the partial slots of each run, cached per map.
 struct SegPlan {
   op_map map = nullptr;
+  int ch = 0;  // cells per workgroup of the update it was made for
   int nseg = 0, nslots = 0;
   int *d_seg_off = nullptr, *d_pslot = nullptr;
   double *d_partials = nullptr;
@@ -1798,10 +1830,10 @@ std::vector<SegPlan> &seg_cache() {
   return cache;
 }
 
-SegPlan &seg_plan(op_map m) {
+SegPlan &seg_plan(op_map m, int ch) {
   std::vector<SegPlan> &cache = seg_cache();
   for (auto &sp : cache)
-    if (sp.map == m) return sp;
+    if (sp.map == m && sp.ch == ch) return sp;
   const int n = m->from->size, nseg = m->to->size;
   const int *mm = map_host(m);
   std::vector<int> off(nseg + 1, 0);
@@ -1811,12 +1843,12 @@ SegPlan &seg_plan(op_map m) {
     off[mm[c] + 1]++;
   }
   for (int i = 0; i < nseg; i++) off[i + 1] += off[i];
-  const int ch = NT * kSegUpc;
   std::vector<int> pslot(nseg + 1, 0);
   for (int i = 0; i < nseg; i++)
     pslot[i + 1] = pslot[i] + (off[i + 1] > off[i] ? (off[i + 1] - 1) / ch - off[i] / ch + 1 : 0);
   SegPlan sp;
   sp.map = m;
+  sp.ch = ch;
   sp.nseg = nseg;
   sp.nslots = pslot[nseg];
   sp.d_seg_off = (int *)dev_alloc(sizeof(int) * off.size());
@@ -1938,7 +1970,7 @@ void prepare_capture() {
   for (auto &up : r.plans) {
     const Plan *p = up.get();
     if (!p->airfoil_res || p->strategy != OP2_STRATEGY_OWNER_GATHER || !p->d_ell ||
-        p->max_owned > 2 * NT)
+        p->max_owned > 2 * block_of(p->loop))
       continue;
     const op_set cells = p->groups[p->owner_group].map->to;
     if ((int)p->owned.size() != cells->size) continue;
@@ -1989,8 +2021,8 @@ void prepare_capture() {
 
 namespace {
 
-#define OP2_UPDS(A, S, B) k_update_seg<T, kSegUpc, A, S, B>
-template <typename T>
+#define OP2_UPDS(A, S, B) k_update_seg<NT, T, kSegUpc, A, S, B>
+template <int NT, typename T>
 auto update_seg_kernel(bool adt, bool save, bool bres) {
   decltype(&OP2_UPDS(false, false, false)) table[8] = {
       OP2_UPDS(false, false, false), OP2_UPDS(false, false, true), OP2_UPDS(false, true, false),
@@ -2000,15 +2032,15 @@ auto update_seg_kernel(bool adt, bool save, bool bres) {
 }
 #undef OP2_UPDS
 
-template <typename T>
+template <int NT, typename T>
 void run_update_batch(op_set set, op_arg *a, const op_arg *adt_a = nullptr,
                       const op_arg *save_a = nullptr, const op_arg *bres_a = nullptr,
                       int nb = 0, bool keep_res = false) {
-  SegPlan &sp = seg_plan(a[4].map);
   const int ch = NT * kSegUpc;
+  SegPlan &sp = seg_plan(a[4].map, ch);
   const int grid = (set->size + ch - 1) / ch;
   Runtime &r = rt();
-  auto kern = update_seg_kernel<T>(adt_a, save_a, bres_a);
+  auto kern = update_seg_kernel<NT, T>(adt_a, save_a, bres_a);
   hipLaunchKernelGGL(kern, dim3(grid), dim3(NT), 0, r.stream,
                      (const T *)a[0].dat->data_d, (T *)a[1].dat->data_d, (T *)a[2].dat->data_d,
                      (const T *)a[3].dat->data_d, set->size, (const int *)a[4].map->map_d,
@@ -2031,12 +2063,12 @@ void run_update_batch(op_set set, op_arg *a, const op_arg *adt_a = nullptr,
     if (r.capturing && r.graph_fusion)
       pending_fold() = PendingFold{true, f};  // the next res_calc carries it
     else
-      hipLaunchKernelGGL(k_fold_seg, dim3(f.nseg), dim3(NT), 0, r.stream, f.partials, f.pslot,
+      hipLaunchKernelGGL(k_fold_seg, dim3(f.nseg), dim3(kFixedBlock), 0, r.stream, f.partials, f.pslot,
                          f.dat, f.log, f.log_base);
   }
 }
 
-template <typename T>
+template <int NT, typename T>
 void run_jac_res(const char *name, op_set set, int nargs, op_arg *a) {
   const int strategy = narrow_strategy(strategy_of(name));
   if (strategy == OP2_STRATEGY_OWNER) {
@@ -2044,9 +2076,9 @@ void run_jac_res(const char *name, op_set set, int nargs, op_arg *a) {
     if (p->arg_slot[1] != 0 || p->arg_slot[2] != 1) fail("op2: unexpected slot layout for %s", name);
     const PlanGroup &nodes = p->groups[0];
     const size_t lds = sizeof(T) * ((size_t)nodes.max_count + p->max_positions);
-    allow_lds(k_jac_res_owner<T>, lds);
+    allow_lds(k_jac_res_owner<NT, T>, lds);
     if (p->nblocks == 0) return;
-    hipLaunchKernelGGL(k_jac_res_owner<T>, dim3(p->nblocks), dim3(NT), lds, rt().stream,
+    hipLaunchKernelGGL((k_jac_res_owner<NT, T>), dim3(p->nblocks), dim3(NT), lds, rt().stream,
                        p->d_blk_off, p->nblocks, p->d_perm, nodes.d_off, nodes.d_list,
                        (const ushort2 *)p->d_loc, p->d_owned_off, p->d_owned, p->d_csr_off,
                        p->d_csr, (const T *)a[0].dat->data_d, (const T *)a[1].dat->data_d,
@@ -2057,7 +2089,7 @@ void run_jac_res(const char *name, op_set set, int nargs, op_arg *a) {
   const bool col = strategy == OP2_STRATEGY_COLOURED;
   const PlanGroup &nodes = p->groups[0];
   const size_t lds = (sizeof(T) * 2 + 1) * (size_t)nodes.max_count;  // us, ds, inc flags
-  auto kern = col ? k_jac_res<T, true> : k_jac_res<T, false>;
+  auto kern = col ? k_jac_res<NT, T, true> : k_jac_res<NT, T, false>;
   if (lds > 64 * 1024)
     OP2_HIP_CHECK(hipFuncSetAttribute((const void *)kern,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
@@ -2075,15 +2107,15 @@ void run_jac_res(const char *name, op_set set, int nargs, op_arg *a) {
 }
 
 // use: as for run_update (two results of kReduceBlocks partials each)
-template <typename T>
+template <int NT, typename T>
 const double *run_jac_update(op_set set, op_arg *a, const ReduceScratch *use = nullptr) {
   ReduceScratch s = use ? *use : reduce_scratch(2);
-  const int grid = grid_for(set->size, kReduceBlocks);
+  const int grid = grid_for(set->size, kReduceBlocks, NT);
   const T alpha = (T)get_const("alpha").as_double(0);
-  hipLaunchKernelGGL(k_jac_update<T>, dim3(grid), dim3(NT), 0, rt().stream,
+  hipLaunchKernelGGL((k_jac_update<NT, T>), dim3(grid), dim3(NT), 0, rt().stream,
                      (const T *)a[0].dat->data_d, (T *)a[1].dat->data_d, (T *)a[2].dat->data_d,
                      set->size, alpha, s.partials);
-  hipLaunchKernelGGL(k_fold<2>, dim3(1), dim3(NT), 0, rt().stream, (const double *)s.partials,
+  hipLaunchKernelGGL((k_fold<2>), dim3(1), dim3(kFixedBlock), 0, rt().stream, (const double *)s.partials,
                      grid, 2u, s.out, rt().deferred ? rt().d_log : nullptr, rt().d_log_count);
   check_launch("update");
   return s.out;
@@ -2159,21 +2191,28 @@ Loop identify(const char *name, op_set set, int nargs, const op_arg *a) {
 
 template <typename T>
 void dispatch(Loop l, const char *name, op_set set, int nargs, op_arg *a) {
-  switch (l) {
-    case Loop::kSaveSoln: run_save_soln<T>(set, a); break;
-    case Loop::kAdtCalc: run_adt_calc<T>(set, a); break;
-    case Loop::kResCalc: run_res_calc<T>(name, set, nargs, a); break;
-    case Loop::kBresCalc: run_bres_calc<T>(name, set, nargs, a); break;
-    case Loop::kJacRes: run_jac_res<T>(name, set, nargs, a); break;
-    default: break;
-  }
+  with_block(block_of(name), [&](auto B) {
+    constexpr int BS = decltype(B)::value;
+    switch (l) {
+      case Loop::kSaveSoln: run_save_soln<BS, T>(set, a); break;
+      case Loop::kAdtCalc: run_adt_calc<BS, T>(set, a); break;
+      case Loop::kResCalc: run_res_calc<BS, T>(name, set, nargs, a); break;
+      case Loop::kBresCalc: run_bres_calc<BS, T>(name, set, nargs, a); break;
+      case Loop::kJacRes: run_jac_res<BS, T>(name, set, nargs, a); break;
+      default: break;
+    }
+  });
 }
 
 // Candidate configurations of a tunable loop, in trial order.
 std::vector<LoopConfig> tune_candidates(Loop l) {
   std::vector<LoopConfig> c;
   if (l == Loop::kResCalc) {
-    for (int part : {128, 256, 512}) c.push_back({OP2_STRATEGY_OWNER_GATHER, part, 0});
+    // owner-gather tiles of 128-512 cells, each with workgroups of 128-512
+    // threads (up to four owned cells per thread)
+    for (int part : {128, 256, 512})
+      for (int block : {128, 256, 512})
+        if (part <= 4 * block) c.push_back({OP2_STRATEGY_OWNER_GATHER, part, block});
     for (int part : {128, 256}) c.push_back({OP2_STRATEGY_OWNER, part, 0});
   } else {
     for (int part : {128, 256, 512}) c.push_back({OP2_STRATEGY_OWNER, part, 0});
@@ -2265,8 +2304,8 @@ void online_call(const std::string &name, OnlineTune &ot, Loop l, int rb, op_set
     ot.events[i].clear();
     if (ot.us[i] < 1e300) ot.us[i] = 1e3 * sum / ot.reps;
     if (OP_diags > 0)
-      printf(" online autotune %s: strategy %d part %d: %.1f us\n", name.c_str(),
-             ot.cands[i].strategy, ot.cands[i].part_size, ot.us[i]);
+      printf(" online autotune %s: strategy %d part %d block %d: %.1f us\n", name.c_str(),
+             ot.cands[i].strategy, ot.cands[i].part_size, ot.cands[i].block_size, ot.us[i]);
     if (ot.us[i] < 1e300 && (bi < 0 || ot.us[i] < ot.us[bi])) bi = i;
   }
   ot.done = true;
@@ -2332,24 +2371,28 @@ void launch_held(Held &h, const op_arg *adt_a, bool keep_res) {
   const op_arg *save_a = h.save_args.empty() ? nullptr : h.save_args.data();
   const op_arg *bres_a = h.bres_args.empty() ? nullptr : h.bres_args.data();
   const int nb = (int)h.bres_args.size();
-  switch (h.l) {
-    case Loop::kBresCalc:
-      run_bres_calc<T>(h.name.c_str(), h.set, (int)h.args.size(), h.args.data());
-      break;
-    case Loop::kSaveSoln:
-      if (adt_a)
-        run_save_adt<T>(h.set, h.args.data(), adt_a);
-      else
-        run_save_soln<T>(h.set, h.args.data());
-      break;
-    case Loop::kUpdate:
-      run_update<T>(h.set, h.args.data(), adt_a, save_a, bres_a, nb, keep_res);
-      break;
-    case Loop::kUpdateBatch:
-      run_update_batch<T>(h.set, h.args.data(), adt_a, save_a, bres_a, nb, keep_res);
-      break;
-    default: fail("op2: loop %s cannot be held for fusion", h.name.c_str());
-  }
+  // the fused kernels run with the held loop's block size
+  with_block(block_of(h.name), [&](auto B) {
+    constexpr int BS = decltype(B)::value;
+    switch (h.l) {
+      case Loop::kBresCalc:
+        run_bres_calc<BS, T>(h.name.c_str(), h.set, (int)h.args.size(), h.args.data());
+        break;
+      case Loop::kSaveSoln:
+        if (adt_a)
+          run_save_adt<BS, T>(h.set, h.args.data(), adt_a);
+        else
+          run_save_soln<BS, T>(h.set, h.args.data());
+        break;
+      case Loop::kUpdate:
+        run_update<BS, T>(h.set, h.args.data(), adt_a, save_a, bres_a, nb, keep_res);
+        break;
+      case Loop::kUpdateBatch:
+        run_update_batch<BS, T>(h.set, h.args.data(), adt_a, save_a, bres_a, nb, keep_res);
+        break;
+      default: fail("op2: loop %s cannot be held for fusion", h.name.c_str());
+    }
+  });
 }
 
 // Launches the held loop, fused with the adt_calc of adt_a if given (or
@@ -2460,8 +2503,9 @@ char *other_ptr(op_dat d) {
 }
 
 template <typename T>
-__global__ __launch_bounds__(NT) void k_copy_elems(const T *__restrict__ src, T *__restrict__ dst,
+__global__ __launch_bounds__(kFixedBlock) void k_copy_elems(const T *__restrict__ src, T *__restrict__ dst,
                                                    long long n) {
+  constexpr int NT = kFixedBlock;
   for (long long i = (long long)blockIdx.x * NT + threadIdx.x; i < n;
        i += (long long)gridDim.x * NT)
     dst[i] = src[i];
@@ -2475,7 +2519,7 @@ void sync_shadows() {
     if (!kv.second) continue;
     op_dat d = kv.first;
     const long long words = (long long)d->size * d->set->size / 4;
-    hipLaunchKernelGGL(k_copy_elems<unsigned>, dim3(grid_for(words, 1 << 16)), dim3(NT), 0,
+    hipLaunchKernelGGL((k_copy_elems<unsigned>), dim3(grid_for(words, 1 << 16)), dim3(kFixedBlock), 0,
                        rt().stream, (const unsigned *)shadow_bufs()[d], (unsigned *)d->data_d,
                        words);
     check_launch("graph stage");
@@ -2493,6 +2537,7 @@ Plan *starts_stage(const char *name, Loop l, int rb, op_set set, int nargs, op_a
   Plan *p = plan_for(name, set, nargs, a, OP2_STRATEGY_OWNER_GATHER, part_of(name, 256));
   if (!bufs_of(p) || !shadow_bufs().count(a[2].dat) || !shadow_bufs().count(a[4].dat))
     return nullptr;
+  if (p->max_owned > 2 * block_of(name)) return nullptr;  // at most two cells per thread
   const op_dat q = a[2].dat, adt = a[4].dat, res = a[6].dat;
   if (q == adt || q == res || adt == res || a[0].dat == q || a[0].dat == adt) return nullptr;
   (void)rb;
@@ -2537,7 +2582,7 @@ bool stage_takes(Loop l, int rb, op_set set, int nargs, op_arg *a) {
   return false;
 }
 
-#define OP2_STG(W, O, B, S, A) k_stage<T, W, O, B, S, A>
+#define OP2_STG(W, O, B, S, A) k_stage<NT, T, W, O, B, S, A>
 #define OP2_STG8(W)                                                                       \
   OP2_STG(W, false, false, false, false), OP2_STG(W, false, false, false, true),           \
       OP2_STG(W, false, false, true, false), OP2_STG(W, false, false, true, true),         \
@@ -2547,7 +2592,7 @@ bool stage_takes(Loop l, int rb, op_set set, int nargs, op_arg *a) {
       OP2_STG(W, true, false, true, false), OP2_STG(W, true, false, true, true),           \
       OP2_STG(W, true, true, false, false), OP2_STG(W, true, true, false, true),           \
       OP2_STG(W, true, true, true, false), OP2_STG(W, true, true, true, true)
-template <typename T>
+template <int NT, typename T>
 auto stage_kernel(int own, bool oid, bool bres, bool save, bool adt) {
   decltype(&OP2_STG(1, true, true, true, true)) table[32] = {OP2_STG8(1), OP2_STG8(2)};
   return table[(own > 1) * 16 + oid * 8 + bres * 4 + save * 2 + adt];
@@ -2555,7 +2600,7 @@ auto stage_kernel(int own, bool oid, bool bres, bool save, bool adt) {
 #undef OP2_STG8
 #undef OP2_STG
 
-template <typename T>
+template <int NT, typename T>
 void run_stage(Stage &st, bool store_res) {
   Runtime &r = rt();
   Plan *p = st.plan;
@@ -2587,8 +2632,9 @@ void run_stage(Stage &st, bool store_res) {
   const int *tiles = st.batch ? tiles_of_meshes(*sb, st.upd[4].map) : nullptr;
   a.nseg = st.batch ? st.upd[4].map->to->size : 0;
   BresIdx<T> bi = has_bres ? bres_idx<T>(st.bres.data(), (int)st.bres.size(), false) : BresIdx<T>{};
-  auto kern = stage_kernel<T>((p->max_owned + NT - 1) / NT, p->owned_identity, has_bres, save,
-                              has_adt);
+  const int own = (p->max_owned + NT - 1) / NT;
+  if (own > 2) fail("op2: stage kernel with %d cells per thread", own);
+  auto kern = stage_kernel<NT, T>(own, p->owned_identity, has_bres, save, has_adt);
   const size_t lds = std::max(sizeof(T) * 4 * (size_t)p->max_positions, (size_t)64);
   allow_lds(kern, lds);
   const CarriedFold fold = take_fold(p->nblocks);
@@ -2629,9 +2675,9 @@ void launch_stage(bool next_takes_res) {
     sync_shadows();
     const int nr = (int)st.res.size();
     if (st.rb == 8)
-      run_res_calc<double>(st.name.c_str(), st.eset, nr, st.res.data());
+      dispatch<double>(Loop::kResCalc, st.name.c_str(), st.eset, nr, st.res.data());
     else
-      run_res_calc<float>(st.name.c_str(), st.eset, nr, st.res.data());
+      dispatch<float>(Loop::kResCalc, st.name.c_str(), st.eset, nr, st.res.data());
     check_launch(st.name.c_str());
     note_writes(nr, st.res.data());
     if (!st.bres.empty()) {
@@ -2639,18 +2685,22 @@ void launch_stage(bool next_takes_res) {
       const int nb = (int)st.bres.size();
       const op_set bset = st.bres[2].map->from;
       if (st.rb == 8)
-        run_bres_calc<double>("bres_calc", bset, nb, st.bres.data());
+        dispatch<double>(Loop::kBresCalc, "bres_calc", bset, nb, st.bres.data());
       else
-        run_bres_calc<float>("bres_calc", bset, nb, st.bres.data());
+        dispatch<float>(Loop::kBresCalc, "bres_calc", bset, nb, st.bres.data());
       check_launch("bres_calc");
       note_writes(nb, st.bres.data());
     }
     return;
   }
-  if (st.rb == 8)
-    run_stage<double>(st, !next_takes_res);
-  else
-    run_stage<float>(st, !next_takes_res);
+  // the stage kernel runs with res_calc's block size
+  with_block(block_of(st.name), [&](auto B) {
+    constexpr int BS = decltype(B)::value;
+    if (st.rb == 8)
+      run_stage<BS, double>(st, !next_takes_res);
+    else
+      run_stage<BS, float>(st, !next_takes_res);
+  });
 }
 }  // namespace
 
@@ -2790,7 +2840,10 @@ int run_loop(const char *name, op_set set, int nargs, op_arg *args) {
   hipEvent_t start;
   timing_begin(t, &start);
   if (l == Loop::kUpdate) {
-    const double *d = rb == 8 ? run_update<double>(set, args) : run_update<float>(set, args);
+    const double *d = with_block(block_of(name), [&](auto B) {
+      constexpr int BS = decltype(B)::value;
+      return rb == 8 ? run_update<BS, double>(set, args) : run_update<BS, float>(set, args);
+    });
     note_writes(nargs, args);
     timing_end(t, start);
     if (rt().deferred) return 0;
@@ -2798,17 +2851,23 @@ int run_loop(const char *name, op_set set, int nargs, op_arg *args) {
     return 0;
   }
   if (l == Loop::kUpdateBatch) {
-    if (rb == 8)
-      run_update_batch<double>(set, args);
-    else
-      run_update_batch<float>(set, args);
+    with_block(block_of(name), [&](auto B) {
+      constexpr int BS = decltype(B)::value;
+      if (rb == 8)
+        run_update_batch<BS, double>(set, args);
+      else
+        run_update_batch<BS, float>(set, args);
+    });
     check_launch(name);
     note_writes(nargs, args);
     timing_end(t, start);
     return 0;
   }
   if (l == Loop::kJacUpdate) {
-    const double *d = rb == 8 ? run_jac_update<double>(set, args) : run_jac_update<float>(set, args);
+    const double *d = with_block(block_of(name), [&](auto B) {
+      constexpr int BS = decltype(B)::value;
+      return rb == 8 ? run_jac_update<BS, double>(set, args) : run_jac_update<BS, float>(set, args);
+    });
     note_writes(nargs, args);
     timing_end(t, start);
     if (rt().deferred) return 0;
@@ -2876,10 +2935,10 @@ int autotune(const char *name, op_set set, int nargs, op_arg *args, int reps, in
   OP2_HIP_CHECK(hipEventCreate(&e0));
   OP2_HIP_CHECK(hipEventCreate(&e1));
   double best = 1e300;
-  int bs = 0, bp = 0;
+  int bs = 0, bp = 0, bb = 0;
   const std::string keep_err = r.last_error;
-  auto trial = [&](int strategy, int part) {
-    r.loop_configs[name] = LoopConfig{strategy, part, 0};
+  auto trial = [&](int strategy, int part, int block) {
+    r.loop_configs[name] = LoopConfig{strategy, part, block};
     const size_t nplans = r.plans.size();
     double us = 1e300;
     try {
@@ -2904,11 +2963,14 @@ int autotune(const char *name, op_set set, int nargs, op_arg *args, int reps, in
       (void)hipGetLastError();
       r.last_error = keep_err;
     }
-    if (OP_diags > 0) printf(" autotune %s: strategy %d part %d: %.1f us\n", name, strategy, part, us);
+    if (OP_diags > 0)
+      printf(" autotune %s: strategy %d part %d block %d: %.1f us\n", name, strategy, part,
+             block_of(name), us);
     if (us < best) {
       best = us;
       bs = strategy;
       bp = part;
+      bb = block;
     }
     // free the plans this trial built unless it is the best so far
     while (r.plans.size() > nplans) {
@@ -2921,9 +2983,12 @@ int autotune(const char *name, op_set set, int nargs, op_arg *args, int reps, in
                             OP2_STRATEGY_COLOURED};
   for (int st : strategies) {
     if (l != Loop::kResCalc && st == OP2_STRATEGY_OWNER_GATHER) continue;
-    trial(st, 256);
+    trial(st, 256, 0);
   }
-  for (int part : {128, 512}) trial(bs, part);
+  for (int part : {128, 512}) trial(bs, part, 0);
+  // then the workgroup size of the best (strategy, part size)
+  for (int block : {128, 256, 512})
+    if (block != OP_block_size) trial(bs, bp, block);
   OP2_HIP_CHECK(hipEventDestroy(e0));
   OP2_HIP_CHECK(hipEventDestroy(e1));
   // drop every losing plan of this loop that is still cached
@@ -2947,7 +3012,7 @@ int autotune(const char *name, op_set set, int nargs, op_arg *args, int reps, in
     r.loop_configs[name] = keep;
     fail("op2: no candidate configuration of loop \"%s\" ran", name);
   }
-  r.loop_configs[name] = LoopConfig{bs, bp, 0};
+  r.loop_configs[name] = LoopConfig{bs, bp, bb};
   if (best_strategy) *best_strategy = bs;
   if (best_part) *best_part = bp;
   if (best_us) *best_us = best;
@@ -3022,33 +3087,31 @@ int par_loop_noalloc(const char *name, op_set set, int nargs, op_arg *args) {
     fail("op2_par_loop_noalloc: loop \"%s\" needs all its real dats to be double or all float",
          name);
   ReduceScratch s = noalloc_scratch();
-  switch (l) {
-    case Loop::kSaveSoln:
-      if (rb == 8) run_save_soln<double>(set, args); else run_save_soln<float>(set, args);
-      check_launch(name);
-      return 0;
-    case Loop::kAdtCalc:
-      if (rb == 8) run_adt_calc<double>(set, args); else run_adt_calc<float>(set, args);
-      check_launch(name);
-      return 0;
-    case Loop::kUpdate: {
-      const double *d = rb == 8
-          ? run_update<double>(set, args, nullptr, nullptr, nullptr, 0, false, &s)
-          : run_update<float>(set, args, nullptr, nullptr, nullptr, 0, false, &s);
-      if (r.deferred) return 0;
-      gbl_inc(args[4], fetch_results(d, 1)[0]);
-      return 0;
-    }
-    default: {
-      const double *d =
-          rb == 8 ? run_jac_update<double>(set, args, &s) : run_jac_update<float>(set, args, &s);
-      if (r.deferred) return 0;
-      const double *h = fetch_results(d, 2);
-      gbl_inc(args[3], h[0]);
-      gbl_max(args[4], h[1]);
-      return 0;
-    }
+  if (l == Loop::kSaveSoln || l == Loop::kAdtCalc) {
+    if (rb == 8)
+      dispatch<double>(l, name, set, nargs, args);
+    else
+      dispatch<float>(l, name, set, nargs, args);
+    check_launch(name);
+    return 0;
   }
+  const double *d = with_block(block_of(name), [&](auto B) {
+    constexpr int BS = decltype(B)::value;
+    if (l == Loop::kUpdate)
+      return rb == 8 ? run_update<BS, double>(set, args, nullptr, nullptr, nullptr, 0, false, &s)
+                     : run_update<BS, float>(set, args, nullptr, nullptr, nullptr, 0, false, &s);
+    return rb == 8 ? run_jac_update<BS, double>(set, args, &s)
+                   : run_jac_update<BS, float>(set, args, &s);
+  });
+  if (r.deferred) return 0;
+  if (l == Loop::kUpdate) {
+    gbl_inc(args[4], fetch_results(d, 1)[0]);
+  } else {
+    const double *h = fetch_results(d, 2);
+    gbl_inc(args[3], h[0]);
+    gbl_max(args[4], h[1]);
+  }
+  return 0;
 }
 
 }  // namespace op2

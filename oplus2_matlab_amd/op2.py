@@ -116,6 +116,8 @@ SIGNATURES = {
     "op2_last_error": (C.c_char_p, []),
     "op2_clear_error": (None, []),
     "op2_set_loop_config": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int]),
+    "op2_get_loop_config": (C.c_int, [C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                      C.POINTER(C.c_int)]),
     "op2_register_loop": (C.c_int, [C.c_char_p, _P]),
     "op2_register_const": (C.c_int, [C.c_char_p, _P]),
     "op2_generic_plan_get": (C.c_int, [C.c_char_p, _P, C.c_int, C.POINTER(OpArg), C.c_int, _P]),
@@ -346,6 +348,14 @@ def set_loop_config(name: str | None, strategy: int = STRATEGY_AUTO, part_size: 
         raise OP2Error(lib().op2_last_error().decode())
 
 
+def get_loop_config(name: str) -> dict:
+    """The configuration loop `name` runs with: strategy (0 = default), part
+    size (0 = default) and block size (threads per workgroup)."""
+    st, part, bs = C.c_int(), C.c_int(), C.c_int()
+    _call("op2_get_loop_config", _b(name), C.byref(st), C.byref(part), C.byref(bs))
+    return {"strategy": st.value, "part_size": part.value, "block_size": bs.value}
+
+
 def loop_stats(name: str):
     """(count, device seconds, algorithmic bytes per call) or None."""
     c, s, b = C.c_int(), C.c_double(), C.c_double()
@@ -374,7 +384,8 @@ def autotune(name: str, set_: Set, args, reps: int = 5) -> dict:
     if _call("op2_autotune_loop", _b(name), set_._p, len(args), arr, reps, C.byref(st),
              C.byref(part), C.byref(us)) != 0:
         raise OP2Error(lib().op2_last_error().decode())
-    return {"strategy": st.value, "part_size": part.value, "us": us.value}
+    return {"strategy": st.value, "part_size": part.value,
+            "block_size": get_loop_config(name)["block_size"], "us": us.value}
 
 
 def checkpoint_record(path: str) -> None:
@@ -434,7 +445,10 @@ def autotune_online_result(name: str):
     rc = _call("op2_autotune_online_result", _b(name), C.byref(st), C.byref(part), C.byref(us))
     if rc < 0:
         raise OP2Error(f"no online tuning of loop {name}")
-    return {"strategy": st.value, "part_size": part.value, "us": us.value} if rc == 1 else None
+    if rc != 1:
+        return None
+    return {"strategy": st.value, "part_size": part.value,
+            "block_size": get_loop_config(name)["block_size"], "us": us.value}
 
 
 def set_deferred_reductions(on: bool) -> bool:

@@ -191,6 +191,24 @@ def test_airfoil_24x12_matches_reference(runtime, golden_dir, strategy, part):
     assert relmax(adt, fx["f64_seq_adt"]) < TOL64
 
 
+@pytest.mark.parametrize("block", [128, 512])
+@pytest.mark.parametrize("strategy", STRATS)
+def test_airfoil_block_sizes_match_reference(runtime, golden_dir, strategy, block):
+    """Every loop's kernels with 128 and 512 threads per workgroup (the
+    indirect loops on 256-element parts: two elements per thread, or half
+    the threads idle in the owner phase) against the reference fixture."""
+    note(f"airfoil fp64 24x12 random, 20 iterations, strategy={strategy} block={block}")
+    op2.set_loop_config(None, 0, 0, block)
+    configure(strategy, 256)
+    assert op2.get_loop_config("res_calc")["block_size"] == block
+    fx = np.load(os.path.join(golden_dir, "airfoil_24x12_random_s1.npz"))
+    mesh = M.o_mesh(24, 12, "random", seed=1)
+    hist, q, adt, res = AF.run_airfoil(mesh, fx["f64_consts"], int(fx["niter"]), "double")
+    assert np.max(np.abs(hist - fx["f64_seq_rms"]) / fx["f64_seq_rms"]) < TOL64
+    assert relmax(q, fx["f64_seq_q"]) < TOL64
+    assert relmax(adt, fx["f64_seq_adt"]) < TOL64
+
+
 @pytest.mark.parametrize("strategy", STRATS)
 def test_airfoil_180k_1000_iterations_matches_reference(runtime, golden_dir, strategy):
     note(f"airfoil fp64 600x300 random, 1000 iterations, strategy={strategy}")
@@ -266,6 +284,21 @@ def test_jacobi_matches_reference(runtime, golden_dir, strategy, part):
     hs, hm, u, du = AF.run_jacobi(prob, int(fx["niter"]), 0.25, -0.25, "double")
     assert relmax(hs, fx["seq_u_sum"]) < TOL64
     assert np.array_equal(hm, fx["seq_u_max"]) or relmax(hm, fx["seq_u_max"]) < TOL64
+    assert relmax(u, fx["seq_u"]) < TOL64
+    assert not du.any()
+
+
+@pytest.mark.parametrize("block", [128, 512])
+@pytest.mark.parametrize("strategy", STRATS)
+def test_jacobi_block_sizes_match_reference(runtime, golden_dir, strategy, block):
+    note(f"jacobi fp64 64x64 random, 200 iterations, strategy={strategy} block={block}")
+    op2.set_loop_config(None, 0, 0, block)
+    configure(strategy, 256)
+    fx = np.load(os.path.join(golden_dir, "jacobi_64_random_s1_f64.npz"))
+    prob = M.jacobi_grid(64, seed=1)
+    hs, hm, u, du = AF.run_jacobi(prob, int(fx["niter"]), 0.25, -0.25, "double")
+    assert relmax(hs, fx["seq_u_sum"]) < TOL64
+    assert relmax(hm, fx["seq_u_max"]) < TOL64
     assert relmax(u, fx["seq_u"]) < TOL64
     assert not du.any()
 

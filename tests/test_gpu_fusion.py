@@ -319,6 +319,56 @@ def test_autotuning_refused_while_graphs_exist(runtime):
     assert got["strategy"] in range(1, 6)
 
 
+@pytest.mark.parametrize("block", [128, 512])
+@pytest.mark.parametrize("dtype", ["double", "float"])
+def test_fused_graphs_at_other_block_sizes(runtime, block, dtype):
+    """Stage kernels and fused update kernels with 128 and 512 threads per
+    workgroup (256-cell tiles: two cells per thread, or half the threads
+    idle in the update phase): fused equals unfused, and FP64 matches the
+    oracle."""
+    note(f"fused graphs with {block}-thread workgroups, 40x20 random renumbered, {dtype}")
+    mesh = M.o_mesh(40, 20, "random", seed=3)
+    c = M.flow_constants().as_array()
+
+    def run(level):
+        op2.set_loop_config(None, 0, 0, block)
+        for loop in ("res_calc", "bres_calc"):
+            op2.set_loop_config(loop, op2.STRATEGY_OWNER_GATHER, 256, 0)
+        return graph_run(mesh, c, dtype, 17, level, True, 4)
+
+    h1, q1, a1, r1 = run(op2.FUSION_STAGES)
+    op2.op_exit()
+    op2.op_init([], 0)
+    h0, q0, a0, r0 = run(op2.FUSION_OFF)
+    assert same_rms(h1, h0, op2.FUSION_STAGES)
+    assert np.array_equal(q1, q0) and np.array_equal(a1, a0) and np.array_equal(r1, r0)
+    if dtype == "double":
+        st, oh = O.airfoil_seq(mesh, c, 17, "f64")
+        assert np.max(np.abs(h1 - oh) / oh) < 1e-12
+
+
+@pytest.mark.parametrize("block", [128, 512])
+def test_fused_batch_at_other_block_sizes(runtime, block):
+    """The batched update's per-mesh partial slots depend on its workgroup
+    size (cells per workgroup): 6 meshes against their oracle runs."""
+    note(f"batch graph with {block}-thread workgroups, 6 meshes, 9 iterations")
+    batch = M.mesh_batch(nmesh=6, seed=4, cells_lo=300, cells_hi=3000, ordering="random")
+    op2.set_loop_config(None, 0, 0, block)
+    for loop in ("res_calc", "bres_calc"):
+        op2.set_loop_config(loop, op2.STRATEGY_OWNER_GATHER, 256, 0)
+    for level in (op2.FUSION_STAGES, op2.FUSION_LOOPS):
+        op2.set_graph_fusion(level)
+        h, _ = AF.run_airfoil_batch(batch, 9, "double", graph=True, graph_iters=4, renumber=True)
+        for i in range(batch.nmesh):
+            _, oh = O.airfoil_seq(batch.meshes[i], batch.consts[i], 9, "f64")
+            assert np.max(np.abs(h[:, i] - oh) / oh) < 1e-12
+        op2.op_exit()
+        op2.op_init([], 0)
+        op2.set_loop_config(None, 0, 0, block)
+        for loop in ("res_calc", "bres_calc"):
+            op2.set_loop_config(loop, op2.STRATEGY_OWNER_GATHER, 256, 0)
+
+
 @pytest.mark.parametrize("dtype", ["double", "float"])
 @pytest.mark.parametrize("renumber", [False, True])
 def test_stage_kernels_with_two_cells_per_thread(runtime, dtype, renumber):
