@@ -239,9 +239,13 @@ def test_airfoil_fp32_form(runtime):
     dq = np.max(np.abs(q - st64.q))
     note(f"fp32 form: rms rel err vs oracle fp32 form {e_mix:.2e}, vs fp64 {e64:.2e}, "
          f"max|q-q64| {dq:.2e}")
-    assert e_mix < 1e-4
-    assert e64 < 1e-4
-    assert dq < 1e-4
+    # owner-gather tiles on the mesh as given add each cell's float
+    # increments in the sequential order, so only the double rms sums differ
+    # (measured 6e-15); an update done in float would differ by ~1e-7
+    assert e_mix < 1e-12
+    # the form's error against FP64 (measured 8.6e-7)
+    assert e64 < 1e-5
+    assert dq < 5e-6
 
 
 @pytest.mark.parametrize("dtype", ["double", "float"])

@@ -39,11 +39,16 @@ def relmax(a, b):
 
 
 def check_meshes(batch, hist, q, niter, form, tol, which=None):
+    """Every mesh's rms history and q against the oracle run of that mesh
+    alone; returns the largest relative errors (rms, q)."""
+    worst = (0.0, 0.0)
     for i in which if which is not None else range(batch.nmesh):
         st, h = O.airfoil_seq(batch.meshes[i], batch.consts[i], niter, form)
         qi = q[batch.cell_off[i]:batch.cell_off[i + 1]]
         eh, eq = relmax(hist[:, i], h), relmax(qi, st.q.reshape(-1, 4))
         assert eh < tol and eq < tol, (i, eh, eq)
+        worst = (max(worst[0], eh), max(worst[1], eq))
+    return worst
 
 
 STRATS = [op2.STRATEGY_ATOMIC, op2.STRATEGY_COLOURED, op2.STRATEGY_DIRECT, op2.STRATEGY_OWNER,
@@ -75,7 +80,10 @@ def test_batch_fp32_form(runtime):
     note("batch fp32 form 4 meshes, 50 iterations")
     batch = M.mesh_batch(nmesh=4, seed=7, cells_lo=500, cells_hi=4000)
     hist, q = AF.run_airfoil_batch(batch, 50, "float", graph=True)
-    check_meshes(batch, hist, q, 50, "mix", 1e-4)
+    # increments in the sequential order (meshes as given): only the double
+    # rms sums differ (measured 4.7e-14, q bit for bit)
+    eh, eq = check_meshes(batch, hist, q, 50, "mix", 1e-12)
+    note(f"batch fp32 form vs oracle fp32 form: rms rel {eh:.2e}, q rel {eq:.2e}")
 
 
 def test_batch_one_mesh_equals_single_program(runtime):

@@ -69,8 +69,9 @@ def test_fp32_form_2400x1200_error_bound(full):
     rel = np.max(np.abs(hist - ohist) / np.abs(ohist))
     dq = float(np.max(np.abs(q.astype(np.float64) - oq)))
     note(f"rms history max rel {rel:.2e}, max |q - q64| {dq:.2e}")
+    # measured 1.2e-6 and 2.5e-5 (|q| ~ 1)
     assert rel < 1e-5
-    assert dq < 1e-3
+    assert dq < 2e-4
 
 
 @pytest.mark.parametrize("renumber", [True, False])

@@ -137,6 +137,13 @@ def test_renumbered_fp32_form(runtime):
     hist, q, _, _ = AF.run_airfoil(mesh, c, 100, "float", graph=True, renumber=True)
     _, hmix = O.airfoil_seq(mesh, c, 100, "mix")
     st64, h64 = O.airfoil_seq(mesh, c, 100, "f64")
-    assert np.max(np.abs(hist - hmix) / np.abs(hmix)) < 1e-4
-    assert np.max(np.abs(hist - h64) / np.abs(h64)) < 1e-4
-    assert np.max(np.abs(q - st64.q)) < 1e-4
+    e_mix = np.max(np.abs(hist - hmix) / np.abs(hmix))
+    e64 = np.max(np.abs(hist - h64) / np.abs(h64))
+    dq = np.max(np.abs(q - st64.q))
+    note(f"renumbered fp32 form: rms rel err vs oracle fp32 form {e_mix:.2e}, vs fp64 {e64:.2e}, "
+         f"max|q-q64| {dq:.2e}")
+    # renumbering changes the order of each cell's float increments (measured
+    # 2.8e-7 against the oracle's FP32 form, 2.0e-7 against FP64)
+    assert e_mix < 5e-6
+    assert e64 < 5e-6
+    assert dq < 5e-6
