@@ -233,10 +233,17 @@ int op2_autotune_online_result(char const *name, int *best_strategy, int *best_p
  * returns data in it, and dats and maps declared later are translated.
  * Plans (and op_plan_get views) use the internal numbering.  Call before
  * capturing graphs; at most once per op_init.  Returns 0.
- * Programs that do not call it get the same with OP2_RENUMBER=auto (or 1,
- * on) in the environment at op_init: the runtime renumbers at the first
- * loop with an indirect OP_INC argument, on that argument's map. */
+ * Programs that do not call it get the same automatically: the runtime
+ * renumbers at the first loop with an indirect OP_INC argument, on that
+ * argument's map, when op2_set_auto_renumber(1) was called before op_init
+ * (the C++ API's op_init of op_lib.h does) or OP2_RENUMBER=auto (1, on) is
+ * in the environment at op_init; OP2_RENUMBER=0 (off, no) keeps the
+ * program's numbering. */
 int op2_renumber(op_map base);
+
+/* Whether op_init arms the automatic renumbering above when OP2_RENUMBER
+ * is not in the environment (default 0).  Returns the previous setting. */
+int op2_set_auto_renumber(int on);
 
 /* The internal number of every element of `set` (new_of_old[set size]; may
  * be NULL): returns 1 if the set was renumbered, 0 if its numbering is the

@@ -44,7 +44,11 @@ extern int OP_diags;
 
 // The program's own -DOP_BLOCK_SIZE / -DOP_PART_SIZE apply first and argv's
 // OP_BLOCK_SIZE= / OP_PART_SIZE= after them (common/op_lib_core.c:67-92).
+// The sets are renumbered for locality at the program's first indirect
+// increment loop (op2_renumber), unless OP2_RENUMBER=0 is in the
+// environment; fetched data keeps the program's numbering.
 inline void op_init(int argc, char **argv, int diags) {
+  op2_set_auto_renumber(1);
 #ifdef OP_BLOCK_SIZE
   OP_block_size = OP_BLOCK_SIZE;
 #endif

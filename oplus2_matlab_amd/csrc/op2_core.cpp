@@ -342,10 +342,15 @@ void op_init_core(int argc, char **argv, int diags) {
   if (OP_block_size != 128 && OP_block_size != 256 && OP_block_size != 512)
     fail("op_init error -- OP_block_size %d is not supported (128, 256 or 512 threads)",
          OP_block_size);
-  // OP2_RENUMBER=1 (or auto, on): renumber without a call in the program
+  // renumber without a call in the program: OP2_RENUMBER=1 (auto, on) or
+  // =0 (off, no) in the environment, else op2_set_auto_renumber's setting
   const char *rn = getenv("OP2_RENUMBER");
-  rt().auto_renumber =
-      rn && (!strcmp(rn, "1") || !strcmp(rn, "auto") || !strcmp(rn, "on"));
+  if (rn && (!strcmp(rn, "1") || !strcmp(rn, "auto") || !strcmp(rn, "on")))
+    rt().auto_renumber = true;
+  else if (rn && (!strcmp(rn, "0") || !strcmp(rn, "off") || !strcmp(rn, "no")))
+    rt().auto_renumber = false;
+  else
+    rt().auto_renumber = rt().auto_renumber_default;
   rt().initialised = true;
   API_END()
 }
@@ -924,6 +929,12 @@ void *op2_device_scratch(size_t bytes) {
   API_BEGIN
   return reduct_scratch(bytes);
   API_END(nullptr)
+}
+
+int op2_set_auto_renumber(int on) {
+  const int old = rt().auto_renumber_default ? 1 : 0;
+  rt().auto_renumber_default = on != 0;
+  return old;
 }
 
 int op2_renumber(op_map base) {

@@ -213,7 +213,8 @@ struct Runtime {
   // set, the starts of its independently ordered pieces, and the
   // renumbered host copies of the maps (op_map::map_i points into them)
   std::map<op_set, std::vector<int>> set_perm;
-  bool auto_renumber = false;  // OP2_RENUMBER set at op_init, not yet applied
+  bool auto_renumber = false;  // renumber at the first indirect increment loop (pending)
+  bool auto_renumber_default = false;  // op2_set_auto_renumber
   std::map<op_set, std::vector<int>> set_segments;
   std::map<op_map, std::vector<int>> map_internal;
   // generic loop path: loops and device constants registered by the

@@ -380,7 +380,7 @@ void auto_renumber(op_set set, int nargs, const op_arg *args) {
         a.map->to == set)
       continue;
     r.auto_renumber = false;
-    if (OP_diags > 1) printf(" renumbering on map %s (OP2_RENUMBER)\n", a.map->name);
+    if (OP_diags > 1) printf(" renumbering on map %s (automatic)\n", a.map->name);
     renumber(a.map);
     return;
   }

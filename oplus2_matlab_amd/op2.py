@@ -129,6 +129,7 @@ SIGNATURES = {
     "op_checkpoint": (C.c_int, []),
     "op2_checkpoint_state": (C.c_int, [C.POINTER(C.c_ulonglong), C.POINTER(C.c_ulonglong)]),
     "op2_renumber": (C.c_int, [_P]),
+    "op2_set_auto_renumber": (C.c_int, [C.c_int]),
     "op2_renumber_export": (C.c_int, [_P, _P]),
     "op2_autotune_online": (C.c_int, [C.c_char_p, C.c_int]),
     "op2_autotune_online_result": (C.c_int, [C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int),

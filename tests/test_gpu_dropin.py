@@ -90,20 +90,42 @@ def test_unchanged_airfoil_program(tmp_path, golden_dir, prog):
             assert gbs2 >= gbs, name
 
 
+@pytest.mark.parametrize("env", [None, "0"])
 @pytest.mark.parametrize("prog", ["airfoil", "airfoil_op"])
-def test_unchanged_airfoil_renumbered_by_the_environment(tmp_path, golden_dir, prog):
-    """OP2_RENUMBER=auto: the same mesh written in a random numbering is
-    renumbered by the runtime at the program's first res_calc (no call in
-    the program) and gives the golden rms history of the cartesian one."""
+def test_unchanged_airfoil_on_a_randomly_numbered_mesh(tmp_path, golden_dir, prog, env):
+    """The same mesh written in a random numbering: renumbered by the
+    runtime at the program's first res_calc (no call in the program), or
+    run as given with OP2_RENUMBER=0; both give the golden rms history of
+    the cartesian one."""
     exe = need(prog)
     M.write_grid_dat(M.o_mesh(120, 60, "random", seed=3), os.path.join(tmp_path, "new_grid.dat"))
-    out = run(exe, tmp_path, env={"OP2_RENUMBER": "auto"})
-    assert "renumbering on map pecell (OP2_RENUMBER)" in out
+    out = run(exe, tmp_path, env={"OP2_RENUMBER": env} if env else None)
+    assert ("renumbering on map pecell (automatic)" in out) == (env is None)
     got = rms_lines(out)
     ref = rms_lines(open(os.path.join(golden_dir, "airfoil_ref_120x60_cartesian.txt")).read())
     assert sorted(got) == sorted(ref) == list(range(100, 1001, 100))
     for it in ref:
         assert abs(got[it] - ref[it]) <= 2e-4 * abs(ref[it]), (it, got[it], ref[it])
+
+
+def test_unchanged_airfoil_random_numbering_runs_near_renumbered_speed(tmp_path):
+    """The unchanged program on a 600 x 300 mesh written in a random
+    numbering: with the default automatic renumbering its res_calc takes
+    within 1.5x of the device time on the cartesian mesh; as given
+    (OP2_RENUMBER=0) it is slower.  Times from the program's own
+    op_timing_output table."""
+    exe = need("airfoil")
+    times = {}
+    for label, ordering, env in (("cartesian", "cartesian", None), ("random", "random", None),
+                                 ("random_as_given", "random", {"OP2_RENUMBER": "0"})):
+        d = tmp_path / label
+        d.mkdir()
+        M.write_grid_dat(M.o_mesh(600, 300, ordering, seed=7), os.path.join(d, "new_grid.dat"))
+        table = timing_table(run(exe, d, env=env))
+        times[label] = table["res_calc"][1] / table["res_calc"][0]
+    print(f"[gpu] res_calc seconds per call {times}", flush=True)
+    assert times["random"] < 1.5 * times["cartesian"]
+    assert times["random_as_given"] > times["random"]
 
 
 @pytest.mark.parametrize("prog", ["jac", "jac_op"])
