@@ -13,12 +13,18 @@ fuses loops that can share a pass: update with the next iteration's
 save_soln and each update or save_soln with the adt_calc that follows
 (--no-fusion captures the loops one kernel each).
 
-roofline: res_calc, the hot loop.  achieved = algorithmic bytes of one call
-(each touched dat once per direction plus each map once: 76 B/edge FP64)
-divided by the mean device time of one call, from HIP events the runtime
-records on its stream around every loop call of an eager timed region.
-traffic = measured HBM bytes per call from the committed rocprofv3 counter
-summary, when this run's configuration is the profiled one.
+roofline: the stage kernel, the dominant kernel of the timed graphs (with
+stage fusion, one kernel per Runge-Kutta stage runs res_calc, bres_calc,
+update and the next save_soln and adt_calc).  achieved = algorithmic bytes
+of its launches (stage_bytes: each map and dat it touches once per
+direction) divided by their device time, from event-record nodes that the
+runtime places around every 7th stage kernel inside the timed graphs
+(op2_set_graph_kernel_timing), read for the last replay of each graph.
+res_calc_roofline: the same for res_calc alone (76 B/edge FP64), from HIP
+events the runtime records on its stream around every loop call of an eager
+timed region.  traffic = measured HBM bytes per launch from the committed
+rocprofv3 counter summary, when this run's configuration is the profiled
+one.
 
 fp32_form: the same mesh with float state and flux (update and rms in
 double): its throughput, speed-up over FP64, and its error against the FP64
@@ -59,6 +65,10 @@ sys.path.insert(0, ROOT)
 
 METRIC = "airfoil cell-updates/s & res_calc HBM GB/s (FP64 vs FP32 form) on 1 MI355X"
 HBM_PEAK_GBS = 8000.0
+# every 7th stage kernel of a timed graph is timed by event-record nodes
+# (each node costs a few microseconds of the replay; an odd interval samples
+# both stages of an iteration)
+MARK_EVERY = 7
 STRATEGIES = {"auto": 0, "coloured": 1, "atomic": 2, "direct": 3, "owner": 4, "owner-gather": 5,
               "tune": -1, "tune-offline": -1}
 
@@ -74,6 +84,8 @@ def parse():
                     help="res_calc strategy; 'tune' runs the runtime's online autotuner in "
                          "the warmup iterations, 'tune-offline' its offline one")
     ap.add_argument("--part", type=int, default=0, help="res_calc part size (0 = default)")
+    ap.add_argument("--block", type=int, default=0,
+                    help="res_calc threads per workgroup with a fixed --strategy (0 = default)")
     ap.add_argument("--renumber", choices=["on", "off"], default="on",
                     help="renumber the sets for locality inside the runtime (op2_renumber)")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-form measurement")
@@ -92,26 +104,35 @@ def parse():
     ap.add_argument("--graph-iters", type=int, default=50,
                     help="iterations per captured graph (at most the step count; a second "
                          "graph runs the remainder)")
+    ap.add_argument("--no-kernel-marks", dest="kernel_marks", action="store_false",
+                    help="capture the timed graphs without the event-record nodes that time "
+                         "their stage kernels")
     return ap.parse_args()
 
 
 TRAFFIC_PROFILE = os.path.join("profiles", "airfoil_2400x1200_fp64_traffic.md")
 
 
-def measured_traffic(ni, nj, res_cfg):
-    """HBM bytes per res_calc launch from the committed rocprofv3 counter
-    summary (FETCH_SIZE x copy-kernel factor + WRITE_SIZE), when this run's
-    configuration is the one profiled (2400x1200 FP64, renumbered,
-    owner-gather with 256-cell tiles); else None."""
+def measured_traffic(ni, nj, res_cfg, keys):
+    """Mean HBM bytes per launch over `keys` (rows of the committed rocprofv3
+    counter summary: FETCH_SIZE x copy-kernel factor + WRITE_SIZE, in MB),
+    when this run is the profiled configuration (2400x1200 FP64, cartesian
+    ordering renumbered, and the res_calc strategy, part and block size on
+    the summary's `config:` line); else None."""
     path = os.path.join(ROOT, TRAFFIC_PROFILE)
     if (ni, nj) != (2400, 1200) or not res_cfg or not os.path.exists(path):
         return None
-    if (res_cfg.get("strategy"), res_cfg.get("part_size")) != (5, 256):
-        return None
+    rows, cfg = {}, None
     for line in open(path):
-        if line.startswith("| `op2::k_res_calc_owner_direct<double"):
-            return float(line.split("|")[7]) * 1e6
-    return None
+        if line.startswith("config:"):
+            cfg = tuple(int(v) for v in line.split()[1:4])
+        elif line.startswith("| key:"):
+            f = [c.strip() for c in line.split("|")]
+            rows[f[1][4:].strip().strip("`")] = float(f[-4]) * 1e6
+    got = (res_cfg.get("strategy"), res_cfg.get("part_size"), res_cfg.get("block_size"))
+    if cfg != got or not keys or any(k not in rows for k in keys):
+        return None
+    return sum(rows[k] for k in keys) / len(keys)
 
 
 def capture_graphs(af, graph_iters, nsteps):
@@ -138,10 +159,11 @@ def warm_graphs(af, k, gid, rid, warmup):
     op2.reduction_log()
 
 
-def measure(mesh, consts, dtype, a, renumber=None, fixed=None, steps=None):
-    """renumber: override of --renumber; fixed: (res_calc strategy, part) instead of
-    --strategy; steps: override of --steps."""
-    """Eager timed region (per-loop HIP events) then graph timed region."""
+def measure(mesh, consts, dtype, a, renumber=None, fixed=None, steps=None, marks=False):
+    """Eager timed region (per-loop HIP events) then graph timed region.
+    renumber: override of --renumber; fixed: (res_calc strategy, part) instead
+    of --strategy; steps: override of --steps; marks: time the graphs' stage
+    kernels with event-record nodes."""
     from oplus2_matlab_amd import airfoil as AF
     from oplus2_matlab_amd import op2
 
@@ -151,8 +173,8 @@ def measure(mesh, consts, dtype, a, renumber=None, fixed=None, steps=None):
     try:
         if fixed:
             op2.set_loop_config("res_calc", fixed[0], fixed[1], 0)
-        elif STRATEGIES[a.strategy] >= 0 and (a.strategy != "auto" or a.part):
-            op2.set_loop_config("res_calc", max(STRATEGIES[a.strategy], 0), a.part, 0)
+        elif STRATEGIES[a.strategy] >= 0 and (a.strategy != "auto" or a.part or a.block):
+            op2.set_loop_config("res_calc", max(STRATEGIES[a.strategy], 0), a.part, a.block)
         af = AF.Airfoil(mesh, consts, dtype, renumber=ren)
         tuned = None
         if not fixed:
@@ -174,7 +196,9 @@ def measure(mesh, consts, dtype, a, renumber=None, fixed=None, steps=None):
                 n, secs, bytes_ = st
                 loops[name] = {"calls": n, "us_per_call": 1e6 * secs / n,
                                "GBs": bytes_ * n / secs / 1e9, "bytes_per_call": bytes_}
+        op2.set_graph_kernel_timing(MARK_EVERY if marks else 0)
         k, gid, rid = capture_graphs(af, a.graph_iters, nsteps)
+        op2.set_graph_kernel_timing(0)
         warm_graphs(af, k, gid, rid, a.warmup)
         t0 = time.perf_counter()
         op2.graph_launch(gid, nsteps // k)
@@ -182,6 +206,10 @@ def measure(mesh, consts, dtype, a, renumber=None, fixed=None, steps=None):
             op2.graph_launch(rid, 1)
         op2.sync()
         dt_graph = time.perf_counter() - t0
+        # the stage kernels of the last replay of each graph (event-record
+        # nodes inside the graphs, on the runtime stream)
+        stages = op2.graph_kernel_times(gid) + (op2.graph_kernel_times(rid) if rid is not None
+                                                else [])
         log = op2.reduction_log()
         rms_last = float((log[-1] / mesh.ncell) ** 0.5) if log.size else None
         op2.set_deferred_reductions(False)
@@ -190,7 +218,49 @@ def measure(mesh, consts, dtype, a, renumber=None, fixed=None, steps=None):
         op2.set_loop_config(None, 0, 0, 0)
     return {"value": mesh.ncell * nsteps / dt_graph, "ms_per_step": 1e3 * dt_graph / nsteps,
             "eager_ms_per_step": 1e3 * dt_eager / nsteps, "loops": loops, "rms_last": rms_last,
-            "tuned": tuned, "graph_iters": k}
+            "tuned": tuned, "graph_iters": k, "stages": stages}
+
+
+def stage_bytes(mesh, kind, size):
+    """Algorithmic bytes of one stage kernel of the given kind (op2 KIND_*
+    bits) with `size`-byte reals: each map and dat it touches once per
+    direction.  Always: pedge and pecell (8 B per edge each), x (2 reals per
+    node), q, adt and qold read and the new q and adt written (4+1+4+4+1
+    reals per cell), and for bres_calc pbedge, pbecell and bound (16 B per
+    boundary edge); SAVE writes qold (4 reals per cell), ADT reads pcell (16
+    B per cell), and res (4 reals per cell) is read and/or stored as the
+    kind says.  The rms partials (8 B per tile) are left out."""
+    from oplus2_matlab_amd import op2
+
+    b = 16 * mesh.nedge + 2 * size * mesh.nnode + 14 * size * mesh.ncell
+    if kind & op2.KIND_BRES:
+        b += 16 * mesh.nbedge
+    if kind & op2.KIND_SAVE:
+        b += 4 * size * mesh.ncell
+    if kind & op2.KIND_ADT:
+        b += 16 * mesh.ncell
+    if kind & op2.KIND_READS_RES:
+        b += 4 * size * mesh.ncell
+    if kind & op2.KIND_STORES_RES:
+        b += 4 * size * mesh.ncell
+    return b
+
+
+def stage_roofline(mesh, stages, size):
+    """Stage kernels of the timed graphs: algorithmic bytes and device time
+    summed over the bracketed launches of each graph's last replay."""
+    if not stages:
+        return None
+    nbytes = sum(stage_bytes(mesh, kind, size) for kind, _ in stages)
+    secs = sum(ms for _, ms in stages) * 1e-3
+    by_kind = {}
+    for kind, ms in stages:
+        by_kind.setdefault(kind, []).append(ms)
+    return {"launches": len(stages), "bytes_per_launch": nbytes / len(stages),
+            "us_per_launch": 1e6 * secs / len(stages), "GBs": nbytes / secs / 1e9,
+            "by_kind": {str(kd): {"launches": len(v), "us_per_launch": 1e3 * sum(v) / len(v),
+                                  "bytes_per_launch": stage_bytes(mesh, kd, size)}
+                        for kd, v in sorted(by_kind.items())}}
 
 
 def measure_batch(a, batch, dtype):
@@ -202,8 +272,8 @@ def measure_batch(a, batch, dtype):
 
     op2.op_init([], 0)
     try:
-        if STRATEGIES[a.strategy] >= 0 and (a.strategy != "auto" or a.part):
-            op2.set_loop_config("res_calc", max(STRATEGIES[a.strategy], 0), a.part, 0)
+        if STRATEGIES[a.strategy] >= 0 and (a.strategy != "auto" or a.part or a.block):
+            op2.set_loop_config("res_calc", max(STRATEGIES[a.strategy], 0), a.part, a.block)
         af = AF.AirfoilBatch(batch, dtype, renumber=a.renumber == "on")
         tuned = (af.autotune(reps=3, online=True) if a.strategy == "tune" else
                  af.autotune() if a.strategy == "tune-offline" else None)
@@ -358,12 +428,33 @@ def main():
     mesh = M.o_mesh(ni, nj, a.ordering, seed=1)
     consts = M.flow_constants().as_array()
 
-    r64 = measure(mesh, consts, "double", a)
+    r64 = measure(mesh, consts, "double", a, marks=a.kernel_marks)
     rc = r64["loops"]["res_calc"]
-    achieved = rc["GBs"]
     res_cfg = (r64["tuned"] or {}).get("res_calc") if a.strategy.startswith("tune") else \
-        {"strategy": STRATEGIES[a.strategy], "part_size": a.part}
-    traffic = measured_traffic(ni, nj, res_cfg) if a.renumber == "on" else None
+        {"strategy": STRATEGIES[a.strategy], "part_size": a.part, "block_size": a.block or 256}
+    profiled = a.renumber == "on" and a.ordering == "cartesian"
+    res_traffic = measured_traffic(ni, nj, res_cfg, ["res_calc"]) if profiled else None
+    res_roof = {"bound": "hbm", "achieved": rc["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": rc["GBs"] / HBM_PEAK_GBS, "traffic": res_traffic,
+                "traffic_source": TRAFFIC_PROFILE if res_traffic else None,
+                "kernel": "res_calc (eager calls)", "bytes_per_launch": rc["bytes_per_call"],
+                "bytes_per_edge": rc["bytes_per_call"] / mesh.nedge,
+                "us_per_launch": rc["us_per_call"]}
+    stage = stage_roofline(mesh, r64["stages"], 8)
+    if stage:
+        traffic = measured_traffic(ni, nj, res_cfg,
+                                   [f"stage/{kd & 7}" for kd, _ in r64["stages"]]) \
+            if profiled else None
+        roofline = {"bound": "hbm", "achieved": stage["GBs"], "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": stage["GBs"] / HBM_PEAK_GBS, "traffic": traffic,
+                    "traffic_source": TRAFFIC_PROFILE if traffic else None,
+                    "kernel": "k_stage (one Runge-Kutta stage: res_calc, bres_calc, update, "
+                              "save_soln, adt_calc), in the timed graphs",
+                    "bytes_per_launch": stage["bytes_per_launch"],
+                    "us_per_launch": stage["us_per_launch"], "launches": stage["launches"],
+                    "by_kind": stage["by_kind"]}
+    else:
+        roofline = res_roof
     result = {
         "metric": METRIC,
         "value": r64["value"],
@@ -385,23 +476,20 @@ def main():
                    "renumbered": a.renumber == "on", "graph_fusion": a.fusion,
                    "graph_iters": r64["graph_iters"],
                    "tuned": r64["tuned"]},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "traffic_source": TRAFFIC_PROFILE if traffic else None,
-                     "kernel": "res_calc", "bytes_per_launch": rc["bytes_per_call"],
-                     "bytes_per_edge": rc["bytes_per_call"] / mesh.nedge,
-                     "us_per_launch": rc["us_per_call"]},
+        "roofline": roofline,
+        "res_calc_roofline": res_roof,
         "eager_ms_per_step": r64["eager_ms_per_step"],
         "loops": r64["loops"],
         "rms_last": r64["rms_last"],
     }
     if not a.no_fp32:
-        r32 = measure(mesh, consts, "float", a)
+        r32 = measure(mesh, consts, "float", a, marks=a.kernel_marks)
         result["fp32_form"] = {"value": r32["value"], "unit": "cell-updates/s",
                                "ms_per_step": r32["ms_per_step"],
                                "speedup_vs_fp64": r32["value"] / r64["value"],
                                "res_calc_GBs": r32["loops"]["res_calc"]["GBs"],
                                "res_calc_us": r32["loops"]["res_calc"]["us_per_call"],
+                               "stage_kernels": stage_roofline(mesh, r32["stages"], 4),
                                "tuned": r32["tuned"],
                                "loops": r32["loops"]}
         if a.accuracy_iters > 0:

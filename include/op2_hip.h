@@ -372,6 +372,23 @@ int op2_graph_destroy(int id);
  * previous level, or -1 during a capture (no change). */
 int op2_set_graph_fusion(int level);
 
+/* Kernel timing inside graphs.  With every = n > 0, the stage kernels
+ * (fusion level 2) of the captures that follow are counted from 1 in each
+ * capture and those numbered n, 2n, 3n, ... are bracketed by two
+ * event-record nodes of their graph (each node costs a few microseconds of
+ * the replay, hence the sampling); 0 turns it off.
+ * op2_graph_kernel_times waits for the runtime stream and returns
+ * the number of bracketed kernels of graph id, filling up to max entries
+ * (launch order) of kinds[] and of ms[] with each one's device time in its
+ * graph's last launch.  kind: 1 the kernel runs bres_calc, 2 the next
+ * save_soln, 4 the adt_calc that follows, 8 it reads res (else res is taken
+ * as zero), 16 it stores res (zeros).  op2_set_graph_kernel_timing returns
+ * the previous setting, or -1 during a capture (no change).  No counterpart
+ * in the reference (its OP_kernels records time whole loops outside
+ * graphs). */
+int op2_set_graph_kernel_timing(int every);
+int op2_graph_kernel_times(int id, int max, int *kinds, float *ms);
+
 /* Device pointer of a dat (for zero-copy interop). */
 void *op2_dat_device_ptr(op_dat dat);
 

@@ -185,6 +185,39 @@ static hipEvent_t take_event() {
   return e;
 }
 
+int mark_begin(int kind) {
+  Runtime &r = rt();
+  if (!r.capturing || r.graph_kernel_timing <= 0) return -1;
+  if (++r.capture_stages % r.graph_kernel_timing) return -1;
+  Runtime::KernelMark m{nullptr, nullptr, kind};
+  for (hipEvent_t *e : {&m.start, &m.stop}) {
+    if (r.mark_pool.empty()) {
+      OP2_HIP_CHECK(hipEventCreate(e));
+    } else {
+      *e = r.mark_pool.back();
+      r.mark_pool.pop_back();
+    }
+  }
+  r.capture_marks.push_back(m);
+  OP2_HIP_CHECK(hipEventRecordWithFlags(m.start, r.stream, hipEventRecordExternal));
+  return (int)r.capture_marks.size() - 1;
+}
+
+void mark_end(int mark) {
+  Runtime &r = rt();
+  if (mark < 0) return;
+  OP2_HIP_CHECK(hipEventRecordWithFlags(r.capture_marks[mark].stop, r.stream,
+                                        hipEventRecordExternal));
+}
+
+static void return_marks(std::vector<Runtime::KernelMark> &marks) {
+  for (auto &m : marks) {
+    rt().mark_pool.push_back(m.start);
+    rt().mark_pool.push_back(m.stop);
+  }
+  marks.clear();
+}
+
 void timing_begin(LoopTiming &t, hipEvent_t *start) {
   (void)t;
   rt().last_plan = nullptr;
@@ -244,6 +277,9 @@ static void clear_all() {
   for (auto &g : r.graphs)
     if (g) (void)hipGraphExecDestroy(g);
   r.graphs.clear();
+  return_marks(r.capture_marks);
+  for (auto &gm : r.graph_marks) return_marks(gm.second);
+  r.graph_marks.clear();
   for (void *p : r.retired) dev_free(p);
   r.retired.clear();
   r.deferred = false;
@@ -759,6 +795,8 @@ int op2_graph_begin(void) {
   timing_drain();
   reduct_scratch(256 + sizeof(double) * 2048 * 4);
   prepare_capture();
+  return_marks(r.capture_marks);
+  r.capture_stages = 0;
   OP2_HIP_CHECK(hipStreamSynchronize(r.stream));
   OP2_HIP_CHECK(hipStreamBeginCapture(r.stream, hipStreamCaptureModeThreadLocal));
   r.capturing = true;
@@ -777,10 +815,13 @@ int op2_graph_end(void) {
     r.capturing = false;
     (void)hipStreamEndCapture(r.stream, &g);
     if (g) (void)hipGraphDestroy(g);
+    return_marks(r.capture_marks);
     throw;
   }
   r.capturing = false;
-  OP2_HIP_CHECK(hipStreamEndCapture(r.stream, &g));
+  const hipError_t ended = hipStreamEndCapture(r.stream, &g);
+  if (ended != hipSuccess) return_marks(r.capture_marks);
+  OP2_HIP_CHECK(ended);
   hipGraphExec_t ex = nullptr;
   OP2_HIP_CHECK(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
   OP2_HIP_CHECK(hipGraphDestroy(g));
@@ -788,7 +829,10 @@ int op2_graph_end(void) {
   // launch
   OP2_HIP_CHECK(hipGraphUpload(ex, r.stream));
   r.graphs.push_back(ex);
-  return (int)r.graphs.size() - 1;
+  const int id = (int)r.graphs.size() - 1;
+  r.graph_marks[id].swap(r.capture_marks);
+  return_marks(r.capture_marks);
+  return id;
   API_END(-1)
 }
 
@@ -816,7 +860,39 @@ int op2_graph_destroy(int id) {
   OP2_HIP_CHECK(hipStreamSynchronize(r.stream));
   OP2_HIP_CHECK(hipGraphExecDestroy(r.graphs[id]));
   r.graphs[id] = nullptr;
+  auto gm = r.graph_marks.find(id);
+  if (gm != r.graph_marks.end()) {
+    return_marks(gm->second);
+    r.graph_marks.erase(gm);
+  }
   return 0;
+  API_END(-1)
+}
+
+int op2_set_graph_kernel_timing(int every) {
+  Runtime &r = rt();
+  const int prev = r.graph_kernel_timing;
+  if (r.capturing) return -1;
+  r.graph_kernel_timing = std::max(every, 0);
+  return prev;
+}
+
+int op2_graph_kernel_times(int id, int max, int *kinds, float *ms) {
+  API_BEGIN
+  Runtime &r = rt();
+  if (id < 0 || id >= (int)r.graphs.size() || !r.graphs[id]) fail("op2: no graph %d", id);
+  if (max < 0 || (max > 0 && (!kinds || !ms)))
+    fail("op2_graph_kernel_times: invalid arguments");
+  if (r.capturing) fail("op2_graph_kernel_times: not during graph capture");
+  auto gm = r.graph_marks.find(id);
+  if (gm == r.graph_marks.end()) return 0;
+  OP2_HIP_CHECK(hipStreamSynchronize(r.stream));
+  const auto &marks = gm->second;
+  for (int i = 0; i < (int)marks.size() && i < max; i++) {
+    kinds[i] = marks[i].kind;
+    OP2_HIP_CHECK(hipEventElapsedTime(&ms[i], marks[i].start, marks[i].stop));
+  }
+  return (int)marks.size();
   API_END(-1)
 }
 

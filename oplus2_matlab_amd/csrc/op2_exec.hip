@@ -2638,9 +2638,12 @@ void run_stage(Stage &st, bool store_res) {
   const size_t lds = std::max(sizeof(T) * 4 * (size_t)p->max_positions, (size_t)64);
   allow_lds(kern, lds);
   const CarriedFold fold = take_fold(p->nblocks);
+  const int mark = mark_begin(has_bres * 1 + save * 2 + has_adt * 4 + !st.zero_in * 8 +
+                              store_res * 16);
   hipLaunchKernelGGL(kern, dim3(p->nblocks), dim3(NT), lds, r.stream, a, bi, airfoil_consts<T>(),
                      fold, st.zero_in, store_res);
   check_launch("graph stage");
+  mark_end(mark);
   in_shadow()[q] = !in_shadow()[q];
   in_shadow()[adt] = !in_shadow()[adt];
   CarriedFold f{};

@@ -209,6 +209,19 @@ struct Runtime {
   std::vector<hipGraphExec_t> graphs;
   std::vector<void *> retired;  // replaced scratch buffers that graphs may use
   std::map<std::string, OnlineTune> online;
+  // kernel timing inside graphs (op2_set_graph_kernel_timing): every n-th
+  // stage kernel of a capture (n = graph_kernel_timing, 0 = none; counted by
+  // capture_stages) is bracketed by two event-record nodes; kind = the loops
+  // it runs and its res traffic (op2_hip.h)
+  struct KernelMark {
+    hipEvent_t start, stop;
+    int kind;
+  };
+  int graph_kernel_timing = 0;
+  int capture_stages = 0;
+  std::vector<hipEvent_t> mark_pool;             // recorded by no live graph
+  std::vector<KernelMark> capture_marks;         // of the capture in progress
+  std::map<int, std::vector<KernelMark>> graph_marks;  // per live graph id
   // renumbering (op2_renumber): new number of every element of a renumbered
   // set, the starts of its independently ordered pieces, and the
   // renumbered host copies of the maps (op_map::map_i points into them)
@@ -234,6 +247,11 @@ void release_exec_caches();
 // cannot allocate), and forgetting a plan's when the plan is dropped
 void prepare_capture();
 void forget_plan_caches(const Plan *p);
+// graph kernel timing: around a kernel launched during capture, mark_begin
+// records the start node (returns -1 when timing is off or not capturing)
+// and mark_end the stop node
+int mark_begin(int kind);
+void mark_end(int mark);
 // loop fusion during graph capture: launch, or forget, a held loop
 void fusion_flush();
 void fusion_drop();

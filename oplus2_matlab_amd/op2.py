@@ -154,6 +154,8 @@ SIGNATURES = {
     "op2_graph_launch": (C.c_int, [C.c_int, C.c_int]),
     "op2_graph_destroy": (C.c_int, [C.c_int]),
     "op2_set_graph_fusion": (C.c_int, [C.c_int]),
+    "op2_set_graph_kernel_timing": (C.c_int, [C.c_int]),
+    "op2_graph_kernel_times": (C.c_int, [C.c_int, C.c_int, _P, _P]),
     "op2_reduction_log_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
 }
 
@@ -507,6 +509,35 @@ def set_graph_fusion(level) -> int:
     if prev < 0:
         raise OP2Error("op2_set_graph_fusion: not during graph capture")
     return prev
+
+
+# kinds of graph_kernel_times: the loops a stage kernel runs besides
+# res_calc and update, and its res traffic
+KIND_BRES, KIND_SAVE, KIND_ADT, KIND_READS_RES, KIND_STORES_RES = 1, 2, 4, 8, 16
+
+
+def set_graph_kernel_timing(every: int) -> int:
+    """With every = n > 0, the stage kernels numbered n, 2n, ... of each
+    capture that follows are bracketed by event-record nodes
+    (graph_kernel_times; each node costs a few microseconds of the replay);
+    0 turns it off.  Returns the previous setting."""
+    prev = _call("op2_set_graph_kernel_timing", max(int(every), 0))
+    if prev < 0:
+        raise OP2Error("op2_set_graph_kernel_timing: not during graph capture")
+    return prev
+
+
+def graph_kernel_times(gid: int) -> list:
+    """[(kind, milliseconds)] of the bracketed kernels of graph gid in its
+    last launch, in launch order (waits for the stream)."""
+    n = _call("op2_graph_kernel_times", gid, 0, None, None)
+    if n < 0:
+        raise OP2Error(lib().op2_last_error().decode())
+    kinds = (C.c_int * max(n, 1))()
+    ms = (C.c_float * max(n, 1))()
+    if _call("op2_graph_kernel_times", gid, n, kinds, ms) < 0:
+        raise OP2Error(lib().op2_last_error().decode())
+    return [(kinds[i], ms[i]) for i in range(n)]
 
 
 def plan_summary(name: str, set_: Set, args, strategy: int = STRATEGY_COLOURED,

@@ -389,3 +389,52 @@ def test_stage_kernels_with_two_cells_per_thread(runtime, dtype, renumber):
     h0, q0, a0, r0 = graph_run(mesh, c, dtype, 11, op2.FUSION_OFF, renumber, 4)
     assert same_rms(h1, h0, op2.FUSION_STAGES)
     assert np.array_equal(q1, q0) and np.array_equal(a1, a0) and np.array_equal(r1, r0)
+
+
+@pytest.mark.parametrize("dtype", ["double", "float"])
+def test_stage_kernel_times_inside_graphs(runtime, dtype):
+    """op2_set_graph_kernel_timing(n) brackets the n-th, 2n-th, ... stage
+    kernel of each capture with event-record nodes.  A 3-iteration graph on a
+    renumbered mesh holds six stage kernels, whose kinds follow the fusion
+    (the first reads res, the second stage of an iteration joins the next
+    save_soln and adt_calc, the last stores res); each bracketed one has a
+    positive device time after a replay; a graph captured with the timing off
+    has none; and the nodes change no result bit."""
+    note(f"graph kernel timing, 60x30 random renumbered, {dtype}")
+    mesh = M.o_mesh(60, 30, "random", seed=5)
+    c = M.flow_constants().as_array()
+    op2.set_graph_kernel_timing(1)
+    try:
+        h1, q1, a1, r1 = graph_run(mesh, c, dtype, 10, op2.FUSION_STAGES, True, 3)
+    finally:
+        op2.set_graph_kernel_timing(0)
+    op2.op_exit()
+    op2.op_init([], 0)
+    h0, q0, a0, r0 = graph_run(mesh, c, dtype, 10, op2.FUSION_STAGES, True, 3)
+    assert np.array_equal(h1, h0)
+    assert np.array_equal(q1, q0) and np.array_equal(a1, a0) and np.array_equal(r1, r0)
+    op2.op_exit()
+    op2.op_init([], 0)
+    af = AF.Airfoil(mesh, c, dtype, renumber=True)
+    af.iterate()
+    op2.set_graph_kernel_timing(1)
+    every = af.capture_iteration(3)
+    op2.set_graph_kernel_timing(2)
+    second = af.capture_iteration(3)
+    op2.set_graph_kernel_timing(0)
+    plain = af.capture_iteration(1)
+    for g in (every, second, plain):
+        op2.graph_launch(g, 2)
+    times = op2.graph_kernel_times(every)
+    note(f"stage kernels {times}")
+    B, S, A, RR, SR = (op2.KIND_BRES, op2.KIND_SAVE, op2.KIND_ADT, op2.KIND_READS_RES,
+                       op2.KIND_STORES_RES)
+    kinds = [B | A | RR, B | S | A, B | A, B | S | A, B | A, B | SR]
+    assert [k for k, _ in times] == kinds
+    assert all(0 < ms < 1000 for _, ms in times)
+    times2 = op2.graph_kernel_times(second)
+    assert [k for k, _ in times2] == kinds[1::2]
+    assert all(0 < ms < 1000 for _, ms in times2)
+    assert op2.graph_kernel_times(plain) == []
+    op2.reduction_log()
+    op2.set_deferred_reductions(False)
