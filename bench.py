@@ -228,8 +228,9 @@ def stage_bytes(mesh, kind, size):
     node), q, adt and qold read and the new q and adt written (4+1+4+4+1
     reals per cell), and for bres_calc pbedge, pbecell and bound (16 B per
     boundary edge); SAVE writes qold (4 reals per cell), ADT reads pcell (16
-    B per cell), and res (4 reals per cell) is read and/or stored as the
-    kind says.  The rms partials (8 B per tile) are left out."""
+    B per cell) or, in the side form, the side recipes (2 B per cell), and
+    res (4 reals per cell) is read and/or stored as the kind says.  The rms
+    partials (8 B per tile) are left out."""
     from oplus2_matlab_amd import op2
 
     b = 16 * mesh.nedge + 2 * size * mesh.nnode + 14 * size * mesh.ncell
@@ -238,7 +239,7 @@ def stage_bytes(mesh, kind, size):
     if kind & op2.KIND_SAVE:
         b += 4 * size * mesh.ncell
     if kind & op2.KIND_ADT:
-        b += 16 * mesh.ncell
+        b += (2 if kind & op2.KIND_ADT_SIDES else 16) * mesh.ncell
     if kind & op2.KIND_READS_RES:
         b += 4 * size * mesh.ncell
     if kind & op2.KIND_STORES_RES:
@@ -443,7 +444,8 @@ def main():
     stage = stage_roofline(mesh, r64["stages"], 8)
     if stage:
         traffic = measured_traffic(ni, nj, res_cfg,
-                                   [f"stage/{kd & 7}" for kd, _ in r64["stages"]]) \
+                                   [f"stage/{kd & 7}" + ("s" if kd & op2.KIND_ADT_SIDES else "")
+                                    for kd, _ in r64["stages"]]) \
             if profiled else None
         roofline = {"bound": "hbm", "achieved": stage["GBs"], "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": stage["GBs"] / HBM_PEAK_GBS, "traffic": traffic,

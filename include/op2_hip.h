@@ -284,6 +284,33 @@ int op2_generic_plan_get(char const *name, op_set set, int nargs, op_arg *args, 
 /* strategy value under which op2_plan_summary_get reports the generic plan */
 #define OP2_PLAN_GENERIC 16
 
+/* Owner-compute plan of a generic loop whose indirect writes are all
+ * OP_INC through one map, at most 4 (map, index) pairs in all: the
+ * incremented targets cut into tiles of `part_size` (0 = the loop's
+ * configured part size, else 128); tile b holds the positions
+ * [blk_off[b], blk_off[b+1]) (perm: the element at each position, every
+ * element incrementing one of the tile's targets, ascending; ids: per
+ * position the target of each (map, index) pair, slot_stride ints), owns
+ * the targets owned_off[b] .. owned_off[b+1]-1 of `owned` (owned = NULL:
+ * the targets with those numbers), and each owned target i lists its
+ * increments csr[csr_off[i] .. csr_off[i+1]-1] = position in the tile << 2
+ * | k, in element order, k = the index of the increment's (map, index)
+ * among the loop's OP_INC pairs (in argument order); when no target has
+ * more than four increments, ell holds the same lists as one 8-byte record
+ * of four entries per owned target (0xffff: none), else NULL.  arg_info[a] = k of
+ * argument a (-1 if it is not an indirect OP_INC) and arg_info[nargs + a] =
+ * its pair's place in an ids row (-1 if it is not indirect).  Device arrays
+ * held by the runtime.  Returns 0, or 1 (nothing filled, no error) when the
+ * loop does not have that form.  No counterpart in the reference (its
+ * plans are coloured). */
+typedef struct {
+  int nblocks, max_positions, max_owned, slot_stride;
+  int const *blk_off, *perm, *ids, *owned_off, *owned, *csr_off;
+  unsigned short const *csr, *ell;
+} op2_generic_owner_plan;
+int op2_generic_owner_plan_get(char const *name, op_set set, int nargs, op_arg *args,
+                               int part_size, op2_generic_owner_plan *out, int *arg_info);
+
 /* Reports an error of a generic loop through the runtime's error mode
  * (prints and exits, or records it for op2_last_error); returns -1. */
 int op2_report_error(char const *msg);
@@ -382,7 +409,9 @@ int op2_set_graph_fusion(int level);
  * (launch order) of kinds[] and of ms[] with each one's device time in its
  * graph's last launch.  kind: 1 the kernel runs bres_calc, 2 the next
  * save_soln, 4 the adt_calc that follows, 8 it reads res (else res is taken
- * as zero), 16 it stores res (zeros).  op2_set_graph_kernel_timing returns
+ * as zero), 16 it stores res (zeros), 32 its adt_calc takes the cells' sides
+ * from the tile's edges and a 2-byte recipe per cell (else it reads pcell
+ * and the nodes).  op2_set_graph_kernel_timing returns
  * the previous setting, or -1 during a capture (no change).  No counterpart
  * in the reference (its OP_kernels records time whole loops outside
  * graphs). */

@@ -23,11 +23,13 @@
  * Execution: every element of the set calls the user function with, per
  * argument, a pointer to the element's record (direct dat), to the record
  * its map points at (indirect dat), or to a thread-local copy (OP_GBL).
- * Loops that write through a map (OP_INC, OP_RW, OP_WRITE, OP_MIN, OP_MAX)
- * run on the runtime's generic plan: blocks of one block colour per launch,
- * and inside a block one thread colour at a time, so no two elements running
- * together write the same target and the user function may update global
- * memory directly.  Other loops run as a grid-stride pass.  OP_GBL
+ * Loops whose indirect writes are all OP_INC through one map run on owner
+ * tiles (below); other loops that write through a map (OP_INC, OP_RW,
+ * OP_WRITE, OP_MIN, OP_MAX) run on the runtime's generic plan: blocks of one
+ * block colour per launch, and inside a block one thread colour at a time,
+ * so no two elements running together write the same target and the user
+ * function may update global memory directly.  Other loops run as a
+ * grid-stride pass.  OP_GBL
  * reductions (OP_INC, OP_MIN, OP_MAX) are combined per workgroup (64-lane
  * shuffles, then LDS, in a fixed order, in double) and folded by one more
  * kernel; the result is added to (INC) or replaces (MIN, MAX, whose start
@@ -63,6 +65,11 @@ struct Arg {
   int bytes;        // dat: bytes per element (dim x type size)
   int kind, dim;    // Kind; OP_GBL: number of components
   int red;          // reduction: first component slot
+  // owner tiles: an OP_INC argument's (map, index) among the loop's OP_INC
+  // pairs, its first argument on the same dat, and the byte offset of its
+  // increments in LDS (-1: not an OP_INC argument); an indirect argument's
+  // (map, index) in the plan's per-position target rows (-1: direct)
+  int inc, leader, lds, slot;
   alignas(16) unsigned char gbl[kGblBytes];  // OP_GBL READ value, MIN/MAX start value
 };
 
@@ -208,6 +215,167 @@ __global__ __launch_bounds__(kThreads) void k_loop(Args<sizeof...(T)> A, int n, 
   if (nred > 0) reduce_all<G, T...>(A, loc, partials, slot, nred, lds, seq);
 }
 
+// ---- owner tiles -------------------------------------------------------------
+// Loops whose indirect writes are all OP_INC through one map, and whose
+// written arguments (non-const pointers of the user function) are exactly
+// those OP_INC dats and OP_GBL MIN/MAX, run on the runtime's owner plan
+// (op2_generic_owner_plan_get) instead of colours.  Each workgroup takes a
+// tile: it evaluates every element that increments one of the tile's
+// targets, the user function incrementing zeroed copies held in registers
+// whose values go to LDS (component-major rows), then each owned target
+// adds its increments in element order and is written once.  No colours,
+// no atomics.  Elements on a tile boundary are evaluated by both of their
+// tiles (each keeps only its own targets' increments), so OP_GBL OP_INC,
+// which would count them twice, keeps the coloured plan.
+
+struct OwnerArrays {
+  int const *blk_off, *perm, *ids, *owned_off, *owned, *csr_off;
+  unsigned short const *csr, *ell;
+  int slot_stride;  // ints per position in ids
+  int stride;       // length of an LDS row: the plan's largest number of positions
+};
+
+template <class T>
+constexpr bool written = !std::is_const<T>::value;
+
+// tile of workgroup wg: each XCD (workgroups wg = x mod 8) takes a
+// contiguous range of tiles, so neighbouring tiles share its L2
+__device__ __forceinline__ int xcd_tile(int wg, int n) {
+  const int x = wg % 8, i = wg / 8, base = n / 8, rem = n % 8;
+  return x * base + (x < rem ? x : rem) + i;
+}
+
+// the record of position pos (global) for the user function: an indirect
+// dat's target from the plan's per-position rows, a direct dat's element
+// through perm, the zeroed copy of an OP_INC dat, or the OP_GBL copy
+template <class T, bool DAT, bool GBL>
+__device__ __forceinline__ T *owner_ptr(const Arg &d, const OwnerArrays &P, int pos,
+                                        Local<T> &gl, Local<T> &inc) {
+  if (GBL) return gl.v;
+  if (DAT && written<T>) return inc.v;
+  const int el = d.slot >= 0 ? P.ids[(size_t)pos * P.slot_stride + d.slot] : P.perm[pos];
+  return reinterpret_cast<T *>(d.data + (size_t)el * d.bytes);
+}
+
+template <class T, bool DAT>
+__device__ __forceinline__ void owner_zero(Local<T> &inc) {
+  if (!(DAT && written<T>)) return;
+  constexpr int n = kGblBytes / sizeof(elem_t<T>);
+#pragma unroll
+  for (int c = 0; c < n; c++) inc.v[c] = elem_t<T>(0);
+}
+
+template <class T, bool DAT>
+__device__ __forceinline__ void owner_store(const Arg &d, const Local<T> &inc, unsigned char *lds,
+                                            int stride, int p) {
+  if (!(DAT && written<T>)) return;
+  constexpr int n = kGblBytes / sizeof(elem_t<T>);
+  elem_t<T> *row = reinterpret_cast<elem_t<T> *>(lds + d.lds);
+#pragma unroll
+  for (int c = 0; c < n; c++)
+    if (c < d.dim) row[c * stride + p] = inc.v[c];
+}
+
+template <auto F, int G, class... T, size_t... I>
+__device__ __forceinline__ void owner_call(const Args<sizeof...(T)> &A, const OwnerArrays &P,
+                                           int pos, std::tuple<Local<T>...> &loc,
+                                           std::tuple<Local<T>...> &inc, unsigned char *lds,
+                                           int p, std::index_sequence<I...>) {
+  constexpr int N = sizeof...(T);
+  (owner_zero<T, ArgClass<I, N, G>::dat>(std::get<I>(inc)), ...);
+  F(owner_ptr<T, ArgClass<I, N, G>::dat, ArgClass<I, N, G>::gbl>(A.a[I], P, pos,
+                                                                  std::get<I>(loc),
+                                                                  std::get<I>(inc))...);
+  (owner_store<T, ArgClass<I, N, G>::dat>(A.a[I], std::get<I>(inc), lds, P.stride, p), ...);
+}
+
+// adds argument J's increment at position pos to v when J writes the dat of
+// argument `leader` through (map, index) pair k
+template <class TI, class TJ, bool DATJ>
+__device__ __forceinline__ void owner_add(const Arg &dj, int leader, int k, int pos,
+                                          const unsigned char *lds, int stride, elem_t<TI> *v,
+                                          int dim) {
+  if (!(DATJ && written<TJ>)) return;
+  if (dj.leader != leader || dj.inc != k) return;
+  constexpr int n = kGblBytes / sizeof(elem_t<TI>);
+  const elem_t<TI> *row = reinterpret_cast<const elem_t<TI> *>(lds + dj.lds);
+#pragma unroll
+  for (int c = 0; c < n; c++)
+    if (c < dim) v[c] += row[c * stride + pos];
+}
+
+// target t of the dat of argument I (the first argument on that dat): its
+// value plus its increments from every argument on the dat, in the order
+// of the list (element order)
+template <int I, int G, class Tup, size_t... J>
+__device__ __forceinline__ void owner_apply(const Args<std::tuple_size<Tup>::value> &A, int t,
+                                            const unsigned short *list, int cnt,
+                                            const unsigned char *lds, int stride,
+                                            std::index_sequence<J...>) {
+  constexpr int N = std::tuple_size<Tup>::value;
+  using TI = std::tuple_element_t<I, Tup>;
+  if (!(ArgClass<I, N, G>::dat && written<TI>)) return;
+  const Arg &d = A.a[I];
+  if (d.leader != I) return;
+  constexpr int n = kGblBytes / sizeof(elem_t<TI>);
+  elem_t<TI> *dst = reinterpret_cast<elem_t<TI> *>(d.data + (size_t)t * d.bytes);
+  elem_t<TI> v[n];
+#pragma unroll
+  for (int c = 0; c < n; c++)
+    if (c < d.dim) v[c] = dst[c];
+  for (int j = 0; j < cnt; j++) {
+    const unsigned e = list[j];
+    if (e == 0xffffu) break;  // (a padded record)
+    const int k = (int)(e & 3u), pos = (int)(e >> 2);
+    (owner_add<TI, std::tuple_element_t<J, Tup>, ArgClass<J, N, G>::dat>(A.a[J], I, k, pos, lds,
+                                                                        stride, v, d.dim),
+     ...);
+  }
+#pragma unroll
+  for (int c = 0; c < n; c++)
+    if (c < d.dim) dst[c] = v[c];
+}
+
+template <int G, class... T, size_t... I>
+__device__ __forceinline__ void owner_apply_all(const Args<sizeof...(T)> &A, int t,
+                                                const unsigned short *list, int cnt,
+                                                const unsigned char *lds, int stride,
+                                                std::index_sequence<I...> seq) {
+  (owner_apply<I, G, std::tuple<T...>>(A, t, list, cnt, lds, stride, seq), ...);
+}
+
+template <auto F, int G, class... T>
+__global__ __launch_bounds__(kThreads) void k_loop_owner(Args<sizeof...(T)> A, int ntiles,
+                                                         int nred, OwnerArrays P,
+                                                         double *partials) {
+  extern __shared__ __align__(16) unsigned char inc_lds[];
+  __shared__ double lds[kThreads / 64];
+  constexpr auto seq = std::index_sequence_for<T...>{};
+  std::tuple<Local<T>...> loc;
+  init_all<G, T...>(A, loc, seq);
+  const int b = xcd_tile(blockIdx.x, ntiles);
+  const int p0 = P.blk_off[b], np = P.blk_off[b + 1] - p0;
+  for (int p = threadIdx.x; p < np; p += kThreads) {
+    std::tuple<Local<T>...> inc;
+    owner_call<F, G, T...>(A, P, p0 + p, loc, inc, inc_lds, p, seq);
+  }
+  __syncthreads();
+  const int o0 = P.owned_off[b], no = P.owned_off[b + 1] - o0;
+  for (int i = threadIdx.x; i < no; i += kThreads) {
+    const int t = P.owned ? P.owned[o0 + i] : o0 + i;
+    if (P.ell) {  // the increments of the target as one 8-byte record
+      const ushort4 r = reinterpret_cast<const ushort4 *>(P.ell)[o0 + i];
+      const unsigned short es[4] = {r.x, r.y, r.z, r.w};
+      owner_apply_all<G, T...>(A, t, es, 4, inc_lds, P.stride, seq);
+    } else {
+      const int j0 = P.csr_off[o0 + i];
+      owner_apply_all<G, T...>(A, t, P.csr + j0, P.csr_off[o0 + i + 1] - j0, inc_lds, P.stride,
+                               seq);
+    }
+  }
+  if (nred > 0) reduce_all<G, T...>(A, loc, partials, b, nred, lds, seq);
+}
+
 // Folds the partials of every reduction component (one workgroup each).
 template <int D = 0>
 __global__ __launch_bounds__(kThreads) void k_loop_fold(const double *partials, int nslots,
@@ -263,6 +431,8 @@ inline int fill(char const *name, const op_arg &g, Arg &d, RedKinds &rk, bool &p
     d.kind = kDat;
     d.data = (char *)op2_dat_device_ptr(g.dat);
     d.bytes = g.dat->size;
+    d.dim = g.dat->dim;
+    d.inc = d.leader = d.lds = d.slot = -1;
     if (!d.data) return error(name, "dat has no device data");
     if (g.map) {
       d.map = op2_map_device_ptr(g.map);
@@ -355,7 +525,57 @@ int run(char const *name, op_set set, int nargs, op_arg *args, std::tuple<T...> 
     partials = (double *)op2_device_scratch(sizeof(double) * ((size_t)slots * rk.n + rk.n));
     return partials ? 0 : -1;
   };
-  if (plan) {
+  // owner tiles: argument classes known at compile time, and every written
+  // argument an indirect OP_INC dat of at most kGblBytes per element or an
+  // OP_GBL MIN/MAX; every other argument read only
+  bool owner = plan && trailing && n > 0;
+  {
+    constexpr bool wr[N] = {written<T>...};
+    for (int i = 0; i < N && owner; i++) {
+      const op_arg &g = args[i];
+      if (g.argtype == OP_ARG_DAT)
+        owner = wr[i] ? (g.map && g.acc == OP_INC && g.dat->size <= kGblBytes) : g.acc == OP_READ;
+      else
+        owner = wr[i] ? (g.acc == OP_MIN || g.acc == OP_MAX) : g.acc == OP_READ;
+    }
+  }
+  op2_generic_owner_plan op;
+  int info[2 * N];  // per argument: its OP_INC pair, its place in a target row
+  if (owner) owner = op2_generic_owner_plan_get(name, set, nargs, args, 0, &op, info) == 0;
+  size_t owner_lds = 0;
+  if (owner) {
+    for (int i = 0; i < N; i++) {
+      Arg &d = A.a[i];
+      if (args[i].argtype != OP_ARG_DAT) continue;
+      d.slot = info[N + i];
+      if (info[i] < 0) continue;
+      d.inc = info[i];
+      d.leader = i;
+      for (int j = 0; j < i && d.leader == i; j++)
+        if (info[j] >= 0 && A.a[j].data == d.data) d.leader = j;
+      d.lds = (int)owner_lds;
+      owner_lds += ((size_t)d.bytes * op.max_positions + 15) / 16 * 16;
+    }
+    owner = owner_lds <= 160 * 1024;
+  }
+  if (owner) {
+    auto okern = G == 0 ? k_loop_owner<F, 0, T...>
+                 : G == 1 ? k_loop_owner<F, (N >= 1 ? 1 : 0), T...>
+                 : G == 2 ? k_loop_owner<F, (N >= 2 ? 2 : 0), T...>
+                 : G == 3 ? k_loop_owner<F, (N >= 3 ? 3 : 0), T...>
+                          : k_loop_owner<F, (N >= 4 ? 4 : 0), T...>;
+    if (scratch(op.nblocks)) return -1;
+    const OwnerArrays P{op.blk_off, op.perm, op.ids,         op.owned_off,    op.owned,
+                        op.csr_off, op.csr,  op.ell,         op.slot_stride, op.max_positions};
+    if (owner_lds > 65536 &&
+        hipFuncSetAttribute(reinterpret_cast<const void *>(okern),
+                            hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)owner_lds) != hipSuccess)
+      return error(name, "cannot raise the LDS limit of the owner-tile kernel");
+    if (op.nblocks > 0)
+      hipLaunchKernelGGL(okern, dim3(op.nblocks), dim3(kThreads), owner_lds, st, A, op.nblocks,
+                         rk.n, P, partials);
+  } else if (plan) {
     op2_generic_plan gp;
     if (op2_generic_plan_get(name, set, nargs, args, 0, &gp) != 0) return -1;
     if (scratch(gp.nblocks)) return -1;

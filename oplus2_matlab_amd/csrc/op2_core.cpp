@@ -978,6 +978,64 @@ int op2_generic_plan_get(char const *name, op_set set, int nargs, op_arg *args, 
   API_END(-1)
 }
 
+int op2_generic_owner_plan_get(char const *name, op_set set, int nargs, op_arg *args,
+                               int part_size, op2_generic_owner_plan *out, int *arg_info) {
+  API_BEGIN
+  if (!name || !set || nargs <= 0 || !args || !out || !arg_info)
+    fail("op2_generic_owner_plan_get: invalid arguments");
+  // the form: indirect writes are OP_INC, all through one map; at most 4
+  // distinct (map, index) pairs
+  op_map inc_map = nullptr;
+  std::vector<std::pair<op_map, int>> pairs;
+  for (int a = 0; a < nargs; a++) {
+    const op_arg &g = args[a];
+    if (g.argtype != OP_ARG_DAT || !g.map) continue;
+    if (g.acc != OP_READ && g.acc != OP_INC) return 1;
+    if (g.acc == OP_INC) {
+      if (inc_map && inc_map != g.map) return 1;
+      inc_map = g.map;
+    }
+    if (std::find(pairs.begin(), pairs.end(), std::make_pair(g.map, g.idx)) == pairs.end())
+      pairs.emplace_back(g.map, g.idx);
+  }
+  if (!inc_map || pairs.size() > 4) return 1;
+  int part = part_size > 0 ? part_size : config_for(name).part_size;
+  if (part <= 0) part = 128;
+  // the gathering form of the owner plan: per position the target of each
+  // (map, index) pair, so that a kernel reads no map
+  PlanRequest req{name, set, nargs, args, OP2_STRATEGY_OWNER_GATHER, part};
+  Plan *p = get_plan(req);
+  p->upload();
+  if (!p->d_perm) {  // (an owner plan of a loop without direct dats has no perm on the device)
+    p->d_perm = (int *)dev_alloc(sizeof(int) * std::max<size_t>(p->perm.size(), 1));
+    h2d(p->d_perm, p->perm.data(), sizeof(int) * p->perm.size());
+    p->device_bytes += sizeof(int) * p->perm.size();
+  }
+  out->nblocks = p->nblocks;
+  out->max_positions = p->max_positions;
+  out->max_owned = p->max_owned;
+  out->slot_stride = p->slot_stride;
+  out->blk_off = p->d_blk_off;
+  out->perm = p->d_perm;
+  out->ids = p->d_ids;
+  out->owned_off = p->d_owned_off;
+  out->owned = p->owned_identity ? nullptr : p->d_owned;
+  out->csr_off = p->d_csr_off;
+  out->csr = p->d_csr;
+  out->ell = p->d_ell;
+  for (int a = 0; a < nargs; a++) {
+    arg_info[a] = arg_info[nargs + a] = -1;
+    if (args[a].argtype != OP_ARG_DAT || !args[a].map) continue;
+    const int s = p->arg_slot[a];
+    arg_info[nargs + a] = s;
+    if (args[a].acc != OP_INC) continue;
+    for (size_t k = 0; k < p->inc_slot_ids.size(); k++)
+      if (p->inc_slot_ids[k] == s) arg_info[a] = (int)k;
+  }
+  return 0;
+  API_END(-1)
+}
+
 int op2_report_error(char const *msg) {
   API_BEGIN
   fail("%s", msg ? msg : "op2: error");

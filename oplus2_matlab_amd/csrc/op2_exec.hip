@@ -896,6 +896,18 @@ __global__ __launch_bounds__(NT) void k_update_seg(const T *qold, T *q,
 // tile's cells is one partial per tile (folded by the next kernel); in the
 // batched form every tile lies in one mesh and the tiles of mesh m are
 // folded in order.
+// adt_calc comes in two forms.  kAdtMap gathers each cell's nodes through
+// adt_calc's pcell.  kAdtSides takes the cell's four sides from the edges
+// the tile has just evaluated: every side of an owned cell is an edge of
+// the tile (one of the cell's increment records) or one of its boundary
+// edges, and a per-cell recipe (3 bits per side, in pcell's side order:
+// 0-3 the increment record, 4-5 the boundary record) says which.  Each
+// thread keeps the side vectors (x1 - x2) of the edges it evaluates in
+// registers and writes them to LDS once the flux sums are read (no more LDS
+// than the fluxes), so adt needs neither pcell nor a second gather of the
+// nodes; the tile's positions must fit in kSideTrips passes of the block.
+constexpr int kAdtNone = 0, kAdtMap = 1, kAdtSides = 2;
+constexpr int kSideTrips = 3;
 template <typename T>
 struct StageArgs {
   const int *blk_off;
@@ -905,18 +917,53 @@ struct StageArgs {
   const ushort4 *ell;
   const T *x, *q_in, *adt_in, *qold;
   T *res, *q_out, *adt_out, *qold_out;
-  const int4 *pcell;  // ADT: the cell -> node map of adt_calc
-  const T *xc;        // ADT: the node coordinates adt_calc reads
+  const int4 *pcell;  // kAdtMap: the cell -> node map of adt_calc
+  const T *xc;        // kAdtMap: the node coordinates adt_calc reads
+  const unsigned short *sides;  // kAdtSides: per owned cell (owned order), its side recipe
   double *partials;   // one per tile
   unsigned *log_count, *log_base;  // batched form: log slots of this stage's nseg results
   int nseg;
 };
 
+// One edge of a stage tile: gathers, flux into LDS (component-major), and
+// the edge's (x1 - x2) for the side form of adt_calc.
+template <typename T>
+__device__ __forceinline__ void stage_edge(const StageArgs<T> &s, const AirfoilConsts<T> &k,
+                                           T *fs, int np, int p0, int p, T side[2]) {
+  const op2_v4i g = __builtin_nontemporal_load((const op2_v4i *)(s.ids + p0 + p));
+  // lower-numbered node and cell first (see k_res_calc_owner_direct)
+  const int na = min(g.x, g.y), nb = max(g.x, g.y);
+  const int ca = min(g.z, g.w), cb = max(g.z, g.w);
+  T xa[2], xb[2], qa[4], qb[4], f[4];
+  Vec<T>::load2(s.x + 2 * (size_t)na, xa);
+  Vec<T>::load2(s.x + 2 * (size_t)nb, xb);
+  Vec<T>::load4(s.q_in + 4 * (size_t)ca, qa);
+  Vec<T>::load4(s.q_in + 4 * (size_t)cb, qb);
+  const T aa = s.adt_in[ca], ab = s.adt_in[cb];
+  const bool sn = g.x != na, sc = g.z != ca;
+  T x1[2], x2[2], q1[4], q2[4];
+#pragma unroll
+  for (int d = 0; d < 2; d++) {
+    x1[d] = sn ? xb[d] : xa[d];
+    x2[d] = sn ? xa[d] : xb[d];
+  }
+#pragma unroll
+  for (int d = 0; d < 4; d++) {
+    q1[d] = sc ? qb[d] : qa[d];
+    q2[d] = sc ? qa[d] : qb[d];
+  }
+  edge_flux(x1, x2, q1, q2, sc ? ab : aa, sc ? aa : ab, k, f);
+#pragma unroll
+  for (int d = 0; d < 4; d++) fs[d * np + p] = f[d];
+  side[0] = x1[0] - x2[0];
+  side[1] = x1[1] - x2[1];
+}
+
 // With one owned cell per thread the register budget is that of 6 waves
 // per SIMD (80 VGPRs): measured faster than the compiler's own choice
 // (about 100 VGPRs, 4 waves) by 7% at 2.88M cells, and than 8 waves (which
 // spills) by 13%.  Two cells per thread keep 4 waves (no spills).
-template <int NT, typename T, int OWN, bool OWNID, bool BRES, bool SAVE, bool ADT>
+template <int NT, typename T, int OWN, bool OWNID, bool BRES, bool SAVE, int ADT>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(OWN == 1 ? 6 : 4, 8))) void
 k_stage(
     StageArgs<T> s, BresIdx<T> bi, AirfoilConsts<T> k, CarriedFold fold, bool zero_res,
@@ -931,55 +978,40 @@ k_stage(
   const int o0 = s.owned_off[b], no = s.owned_off[b + 1] - o0;
   T *fs = reinterpret_cast<T *>(smem);  // [4][np]
   const int tid = threadIdx.x;
-  // the owned cells' increment records and (unless res is known zero) res,
-  // issued before the flux phase so that their latency overlaps it
+  // the owned cells' increment records (and side recipes), issued before
+  // the flux phase so that their latency overlaps it
   bool own[OWN];
   int c[OWN];
   ushort4 e4[OWN];
-  T rpre[OWN][4];
+  unsigned recipe[OWN];
 #pragma unroll
   for (int j = 0; j < OWN; j++) {
     const int i = tid + j * NT;
     own[j] = i < no;
     c[j] = own[j] ? (OWNID ? o0 + i : s.owned[o0 + i]) : 0;
     e4[j] = make_ushort4(0xffff, 0xffff, 0xffff, 0xffff);
-#pragma unroll
-    for (int d = 0; d < 4; d++) rpre[j][d] = T(0);
+    recipe[j] = 0;
     if (own[j]) {
       const op2_v4us v = __builtin_nontemporal_load((const op2_v4us *)(s.ell + o0 + i));
       e4[j] = make_ushort4(v.x, v.y, v.z, v.w);
-      if (!zero_res) Vec<T>::load4(s.res + 4 * (size_t)c[j], rpre[j]);
+      if (ADT == kAdtSides) recipe[j] = s.sides[o0 + i];
     }
   }
-  for (int p = tid; p < np; p += NT) {
-    const op2_v4i g = __builtin_nontemporal_load((const op2_v4i *)(s.ids + p0 + p));
-    // lower-numbered node and cell first (see k_res_calc_owner_direct)
-    const int na = min(g.x, g.y), nb = max(g.x, g.y);
-    const int ca = min(g.z, g.w), cb = max(g.z, g.w);
-    T xa[2], xb[2], qa[4], qb[4], f[4];
-    Vec<T>::load2(s.x + 2 * (size_t)na, xa);
-    Vec<T>::load2(s.x + 2 * (size_t)nb, xb);
-    Vec<T>::load4(s.q_in + 4 * (size_t)ca, qa);
-    Vec<T>::load4(s.q_in + 4 * (size_t)cb, qb);
-    const T aa = s.adt_in[ca], ab = s.adt_in[cb];
-    const bool sn = g.x != na, sc = g.z != ca;
-    T x1[2], x2[2], q1[4], q2[4];
+  T side[kSideTrips][2];
+  if (ADT == kAdtSides) {
 #pragma unroll
-    for (int d = 0; d < 2; d++) {
-      x1[d] = sn ? xb[d] : xa[d];
-      x2[d] = sn ? xa[d] : xb[d];
+    for (int t = 0; t < kSideTrips; t++) {
+      const int p = tid + t * NT;
+      if (p < np) stage_edge(s, k, fs, np, p0, p, side[t]);
     }
-#pragma unroll
-    for (int d = 0; d < 4; d++) {
-      q1[d] = sc ? qb[d] : qa[d];
-      q2[d] = sc ? qa[d] : qb[d];
+  } else {
+    for (int p = tid; p < np; p += NT) {
+      T unused[2];
+      stage_edge(s, k, fs, np, p0, p, unused);
     }
-    edge_flux(x1, x2, q1, q2, sc ? ab : aa, sc ? aa : ab, k, f);
-#pragma unroll
-    for (int d = 0; d < 4; d++) fs[d * np + p] = f[d];
   }
   // the owned cells' update inputs (after the flux phase: fewer registers
-  // held through it) and, for the new adt, the nodes of each cell, gathered
+  // held through it) and, for kAdtMap, the nodes of each cell, gathered
   // while the other waves finish their fluxes
   T qo[OWN][4], a[OWN], xs[OWN][4][2];
   int bidx[OWN];
@@ -995,19 +1027,23 @@ k_stage(
       const unsigned bit = 1u << (c[j] & 31);
       bidx[j] = (w.x & bit) ? (int)(w.y + __popc(w.x & (bit - 1u))) : -1;
     }
-    if (ADT) cell_nodes(s.pcell, s.xc, c[j], xs[j]);
+    if (ADT == kAdtMap) cell_nodes(s.pcell, s.xc, c[j], xs[j]);
   }
   __syncthreads();
   double acc = 0.0;
+  T qk[OWN][4];  // kAdtSides: the new q, for adt after the side exchange
 #pragma unroll
   for (int j = 0; j < OWN; j++) {
     if (!own[j]) continue;
     const int cj = c[j];
     T sum[4] = {T(0), T(0), T(0), T(0)};
     ell_sum(fs, np, e4[j], sum);
-    T r[4];
+    // res (+0 when it is known zero, the first stage of a graph excepted:
+    // read here, not held in registers through the flux phase)
+    T r[4] = {T(0), T(0), T(0), T(0)};
+    if (!zero_res) Vec<T>::load4(s.res + 4 * (size_t)cj, r);
 #pragma unroll
-    for (int d = 0; d < 4; d++) r[d] = rpre[j][d] + sum[d];
+    for (int d = 0; d < 4; d++) r[d] = r[d] + sum[d];
     if (BRES && bidx[j] >= 0) {
       T qc[4];
       Vec<T>::load4(s.q_in + 4 * (size_t)cj, qc);
@@ -1034,10 +1070,60 @@ k_stage(
     }
     Vec<T>::store4(s.q_out + 4 * (size_t)cj, qn);
     if (SAVE) Vec<T>::store4(s.qold_out + 4 * (size_t)cj, qn);
-    s.adt_out[cj] = ADT ? adt_of(xs[j][0], xs[j][1], xs[j][2], xs[j][3], qn, k) : a[j];
+    if (ADT == kAdtMap) s.adt_out[cj] = adt_of(xs[j][0], xs[j][1], xs[j][2], xs[j][3], qn, k);
+    if (ADT == kAdtNone) s.adt_out[cj] = a[j];
+#pragma unroll
+    for (int d = 0; d < 4; d++) qk[j][d] = qn[d];
     if (store_res) {
       const T z[4] = {T(0), T(0), T(0), T(0)};
       Vec<T>::store4(s.res + 4 * (size_t)cj, z);
+    }
+  }
+  if (ADT == kAdtSides) {
+    // every flux sum is read: the flux rows take the side vectors
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < kSideTrips; t++) {
+      const int p = tid + t * NT;
+      if (p < np) {
+        fs[p] = side[t][0];
+        fs[np + p] = side[t][1];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < OWN; j++) {
+      if (!own[j]) continue;
+      // every side read from LDS without branches (a boundary side reads
+      // position 0 and is replaced below: only boundary cells branch)
+      T dxs[4], dys[4];
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const unsigned src = (recipe[j] >> (3 * i)) & 7u;
+        // (selects, not an indexed array: that would live in scratch)
+        const unsigned e = (src & 3u) == 0   ? e4[j].x
+                           : (src & 3u) == 1 ? e4[j].y
+                           : (src & 3u) == 2 ? e4[j].z
+                                             : e4[j].w;
+        const int pos = src < 4 ? (int)(e >> 2) : 0;
+        dxs[i] = fs[pos];
+        dys[i] = fs[np + pos];
+      }
+      if (BRES && bidx[j] >= 0) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          const unsigned src = (recipe[j] >> (3 * i)) & 7u;
+          if (src < 4) continue;
+          // a boundary edge of the cell (nodes from its record)
+          const int4 rec = bi.rec[2 * bidx[j] + (int)(src - 4)];
+          T x1[2], x2[2];
+          Vec<T>::load2(bi.x + 2 * (size_t)rec.x, x1);
+          Vec<T>::load2(bi.x + 2 * (size_t)rec.y, x2);
+          dxs[i] = x1[0] - x2[0];
+          dys[i] = x1[1] - x2[1];
+        }
+      }
+      s.adt_out[c[j]] = adt_of_sides(dxs, dys, qk[j], k);
     }
   }
   acc = block_reduce<kRedSum, NT>(acc, red);
@@ -1872,7 +1958,14 @@ namespace {
 // per-tile rms partials and of batched log slots (a stage writes one set
 // while its workgroups fold the previous stage's), the single form's result
 // slot, and for each cell -> mesh map of a batched update the first tile of
-// each mesh (empty when a tile holds cells of two meshes).
+// each mesh (empty when a tile holds cells of two meshes).  Side recipes
+// (kAdtSides) per (adt_calc cell -> node map, bres_calc maps): one 16-bit
+// recipe per owned cell, or none when some side of some cell is neither an
+// edge of its increment records nor one of its boundary edges.
+struct SideRecipe {
+  op_map pcell = nullptr, pbedge = nullptr, pbecell = nullptr, bmesh = nullptr;
+  unsigned short *d = nullptr;
+};
 struct StageBufs {
   const Plan *plan = nullptr;
   double *partials[2] = {nullptr, nullptr};
@@ -1880,6 +1973,7 @@ struct StageBufs {
   double *out = nullptr;
   int parity = 0;
   std::vector<std::pair<op_map, int *>> tile_off;
+  std::vector<SideRecipe> sides;
 };
 
 std::vector<StageBufs> &stage_bufs() {
@@ -1925,6 +2019,67 @@ void free_stage_bufs(StageBufs &b, std::vector<void *> *keep) {
   }
   drop(b.out);
   for (auto &t : b.tile_off) drop(t.second);
+  for (auto &s : b.sides) drop(s.d);
+}
+
+// The side recipe of every owned cell of owner plan p (res_calc: args 0
+// and 1 are the edge's nodes through pedge) for adt_calc's pcell and the
+// boundary edges of a bres_calc on pbedge / pbecell: side i of cell c runs
+// from pcell node i to node i+1 (mod 4); its 3-bit source is the first
+// unused increment record of the cell whose edge has those two nodes (0-3),
+// else the first unused boundary edge of the cell, in edge order, with those
+// nodes (4-5: the cell's boundary record).  Empty when a side has no source.
+std::vector<unsigned short> side_recipes(const Plan *p, op_map pcell, op_map pbedge,
+                                         op_map pbecell) {
+  std::vector<unsigned short> out;
+  const op_map pedge = p->maps[0];
+  if (p->ell.empty() || !pedge || p->maps[1] != pedge || pcell->dim != 4 || pbedge->dim < 2)
+    return out;
+  if (p->perm.empty() || pbecell->dim != 1 || pcell->from != pbecell->to) return out;
+  const int nc = pcell->from->size;
+  const int *en = map_host(pedge), *cn = map_host(pcell), *bn = map_host(pbedge),
+            *bc = map_host(pbecell);
+  const int i0 = p->idxs[0], i1 = p->idxs[1];
+  // boundary edges of each cell in edge order (the order of the boundary
+  // records, build_bres_index)
+  std::vector<int> boff(nc + 1, 0);
+  for (int e = 0; e < pbecell->from->size; e++) boff[bc[e] + 1]++;
+  for (int c = 0; c < nc; c++) boff[c + 1] += boff[c];
+  std::vector<int> bedges(boff[nc]);
+  {
+    std::vector<int> fill(boff.begin(), boff.end() - 1);
+    for (int e = 0; e < pbecell->from->size; e++) bedges[fill[bc[e]]++] = e;
+  }
+  auto same = [](int a, int b, int u, int v) { return (a == u && b == v) || (a == v && b == u); };
+  out.assign(p->owned.size(), 0);
+  for (int t = 0; t < p->nblocks; t++)
+    for (int i = p->owned_off[t]; i < p->owned_off[t + 1]; i++) {
+      const int c = p->owned[i];
+      if (boff[c + 1] - boff[c] > 2) return {};
+      unsigned used = 0, recipe = 0;
+      for (int k = 0; k < 4; k++) {
+        const int u = cn[4 * (size_t)c + k], v = cn[4 * (size_t)c + (k + 1) % 4];
+        int src = -1;
+        for (int m = 0; m < 4 && src < 0; m++) {
+          const unsigned short e = p->ell[4 * (size_t)i + m];
+          if (e == 0xffff || (used >> m & 1)) continue;
+          const int edge = p->perm[p->blk_off[t] + (e >> 2)];
+          if (same(en[pedge->dim * (size_t)edge + i0], en[pedge->dim * (size_t)edge + i1], u, v))
+            src = m;
+        }
+        for (int m = 0; m < boff[c + 1] - boff[c] && src < 0; m++) {
+          const int be = bedges[boff[c] + m];
+          if (!(used >> (4 + m) & 1) &&
+              same(bn[pbedge->dim * (size_t)be], bn[pbedge->dim * (size_t)be + 1], u, v))
+            src = 4 + m;
+        }
+        if (src < 0) return {};
+        used |= 1u << src;
+        recipe |= (unsigned)src << (3 * k);
+      }
+      out[i] = (unsigned short)recipe;
+    }
+  return out;
 }
 
 }  // namespace
@@ -2015,6 +2170,27 @@ void prepare_capture() {
         h2d(d, first.data(), sizeof(int) * first.size());
       }
       b->tile_off.emplace_back(sp.map, d);
+    }
+    // side recipes for each cell -> 4 nodes map and each boundary index on
+    // these cells (adt_calc and bres_calc joining this plan's stages)
+    for (int mi = 0; mi < OP_map_index; mi++) {
+      op_map pc = OP_map_list[mi];
+      if (pc->from != cells || pc->dim != 4) continue;
+      for (auto &bx : bres_cache()) {
+        if (!bx.usable || bx.pbecell->to != cells) continue;
+        bool known = false;
+        for (auto &s : b->sides)
+          known |= s.pcell == pc && s.pbedge == bx.pbedge && s.pbecell == bx.pbecell &&
+                   s.bmesh == bx.bmesh;
+        if (known) continue;
+        SideRecipe sr{pc, bx.pbedge, bx.pbecell, bx.bmesh, nullptr};
+        const std::vector<unsigned short> h = side_recipes(p, pc, bx.pbedge, bx.pbecell);
+        if (!h.empty()) {
+          sr.d = (unsigned short *)dev_alloc(sizeof(unsigned short) * h.size());
+          h2d(sr.d, h.data(), sizeof(unsigned short) * h.size());
+        }
+        b->sides.push_back(sr);
+      }
     }
   }
 }
@@ -2209,10 +2385,14 @@ std::vector<LoopConfig> tune_candidates(Loop l) {
   std::vector<LoopConfig> c;
   if (l == Loop::kResCalc) {
     // owner-gather tiles of 128-512 cells, each with workgroups of 128-512
-    // threads (up to four owned cells per thread)
+    // threads holding one owned cell each: the form in which the stage
+    // kernels of captured graphs run fastest (two cells per thread keep 4
+    // waves per SIMD and, with more positions than three passes of the
+    // workgroup, take adt_calc's nodes through pcell: FP32 stage 86 against
+    // 78 us at 2.88M cells)
     for (int part : {128, 256, 512})
       for (int block : {128, 256, 512})
-        if (part <= 4 * block) c.push_back({OP2_STRATEGY_OWNER_GATHER, part, block});
+        if (part <= block) c.push_back({OP2_STRATEGY_OWNER_GATHER, part, block});
     for (int part : {128, 256}) c.push_back({OP2_STRATEGY_OWNER, part, 0});
   } else {
     for (int part : {128, 256, 512}) c.push_back({OP2_STRATEGY_OWNER, part, 0});
@@ -2583,21 +2763,19 @@ bool stage_takes(Loop l, int rb, op_set set, int nargs, op_arg *a) {
 }
 
 #define OP2_STG(W, O, B, S, A) k_stage<NT, T, W, O, B, S, A>
-#define OP2_STG8(W)                                                                       \
-  OP2_STG(W, false, false, false, false), OP2_STG(W, false, false, false, true),           \
-      OP2_STG(W, false, false, true, false), OP2_STG(W, false, false, true, true),         \
-      OP2_STG(W, false, true, false, false), OP2_STG(W, false, true, false, true),         \
-      OP2_STG(W, false, true, true, false), OP2_STG(W, false, true, true, true),           \
-      OP2_STG(W, true, false, false, false), OP2_STG(W, true, false, false, true),         \
-      OP2_STG(W, true, false, true, false), OP2_STG(W, true, false, true, true),           \
-      OP2_STG(W, true, true, false, false), OP2_STG(W, true, true, false, true),           \
-      OP2_STG(W, true, true, true, false), OP2_STG(W, true, true, true, true)
+#define OP2_STG3(W, O, B, S) OP2_STG(W, O, B, S, 0), OP2_STG(W, O, B, S, 1), OP2_STG(W, O, B, S, 2)
+#define OP2_STG24(W)                                                                      \
+  OP2_STG3(W, false, false, false), OP2_STG3(W, false, false, true),                       \
+      OP2_STG3(W, false, true, false), OP2_STG3(W, false, true, true),                     \
+      OP2_STG3(W, true, false, false), OP2_STG3(W, true, false, true),                     \
+      OP2_STG3(W, true, true, false), OP2_STG3(W, true, true, true)
 template <int NT, typename T>
-auto stage_kernel(int own, bool oid, bool bres, bool save, bool adt) {
-  decltype(&OP2_STG(1, true, true, true, true)) table[32] = {OP2_STG8(1), OP2_STG8(2)};
-  return table[(own > 1) * 16 + oid * 8 + bres * 4 + save * 2 + adt];
+auto stage_kernel(int own, bool oid, bool bres, bool save, int adt) {
+  decltype(&OP2_STG(1, true, true, true, 0)) table[48] = {OP2_STG24(1), OP2_STG24(2)};
+  return table[(own > 1) * 24 + oid * 12 + bres * 6 + save * 3 + adt];
 }
-#undef OP2_STG8
+#undef OP2_STG24
+#undef OP2_STG3
 #undef OP2_STG
 
 template <int NT, typename T>
@@ -2634,12 +2812,28 @@ void run_stage(Stage &st, bool store_res) {
   BresIdx<T> bi = has_bres ? bres_idx<T>(st.bres.data(), (int)st.bres.size(), false) : BresIdx<T>{};
   const int own = (p->max_owned + NT - 1) / NT;
   if (own > 2) fail("op2: stage kernel with %d cells per thread", own);
-  auto kern = stage_kernel<NT, T>(own, p->owned_identity, has_bres, save, has_adt);
+  // adt_calc from the tile's edges when a side recipe exists for these maps,
+  // every node is read from the one x dat, and the tile's positions fit the
+  // kernel's passes; else through pcell
+  int adt_mode = has_adt ? kAdtMap : kAdtNone;
+  if (has_adt && has_bres && p->max_positions <= kSideTrips * NT &&
+      st.adt[0].dat == st.res[0].dat && st.bres[0].dat == st.res[0].dat) {
+    bool in_order = true;
+    for (int i = 0; i < 4; i++) in_order &= st.adt[i].map == st.adt[0].map && st.adt[i].idx == i;
+    const op_map bm = st.bres.size() == 7 ? st.bres[6].map : nullptr;
+    for (auto &sr : sb->sides)
+      if (in_order && sr.d && sr.pcell == st.adt[0].map && sr.pbedge == st.bres[0].map &&
+          sr.pbecell == st.bres[2].map && sr.bmesh == bm) {
+        a.sides = sr.d;
+        adt_mode = kAdtSides;
+      }
+  }
+  auto kern = stage_kernel<NT, T>(own, p->owned_identity, has_bres, save, adt_mode);
   const size_t lds = std::max(sizeof(T) * 4 * (size_t)p->max_positions, (size_t)64);
   allow_lds(kern, lds);
   const CarriedFold fold = take_fold(p->nblocks);
   const int mark = mark_begin(has_bres * 1 + save * 2 + has_adt * 4 + !st.zero_in * 8 +
-                              store_res * 16);
+                              store_res * 16 + (adt_mode == kAdtSides) * 32);
   hipLaunchKernelGGL(kern, dim3(p->nblocks), dim3(NT), lds, r.stream, a, bi, airfoil_consts<T>(),
                      fold, st.zero_in, store_res);
   check_launch("graph stage");

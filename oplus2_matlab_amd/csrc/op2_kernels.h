@@ -101,6 +101,25 @@ __device__ __forceinline__ T adt_of(const T x1[2], const T x2[2], const T x3[2],
   return s / k.cfl;
 }
 
+// adt_of with the cell's four side vectors given, side i from node i to node
+// i+1 (mod 4), each possibly reversed: a reversed (dx, dy) is the exact
+// negation (x_a - x_b = -(x_b - x_a) in IEEE arithmetic), which leaves
+// fabs(u dy - v dx) and dx dx + dy dy, and so every bit of the result,
+// unchanged.
+template <typename T>
+__device__ __forceinline__ T adt_of_sides(const T dxs[4], const T dys[4], const T q[4],
+                                          const AirfoilConsts<T> &k) {
+  T ri = T(1) / q[0];
+  T u = ri * q[1];
+  T v = ri * q[2];
+  T c = sqrt(k.gam * k.gm1 * (ri * q[3] - T(0.5) * (u * u + v * v)));
+  T s = fabs(u * dys[0] - v * dxs[0]) + c * sqrt(dxs[0] * dxs[0] + dys[0] * dys[0]);
+#pragma unroll
+  for (int i = 1; i < 4; i++)
+    s += fabs(u * dys[i] - v * dxs[i]) + c * sqrt(dxs[i] * dxs[i] + dys[i] * dys[i]);
+  return s / k.cfl;
+}
+
 // airfoil/res_calc.h:1-30: f is added to the first cell, subtracted from the
 // second.
 template <typename T>

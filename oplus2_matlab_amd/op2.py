@@ -121,6 +121,8 @@ SIGNATURES = {
     "op2_register_loop": (C.c_int, [C.c_char_p, _P]),
     "op2_register_const": (C.c_int, [C.c_char_p, _P]),
     "op2_generic_plan_get": (C.c_int, [C.c_char_p, _P, C.c_int, C.POINTER(OpArg), C.c_int, _P]),
+    "op2_generic_owner_plan_get": (C.c_int, [C.c_char_p, _P, C.c_int, C.POINTER(OpArg), C.c_int,
+                                             _P, _P]),
     "op2_map_device_ptr": (_P, [_P]),
     "op2_device_scratch": (_P, [C.c_size_t]),
     "op2_report_error": (C.c_int, [C.c_char_p]),
@@ -512,8 +514,9 @@ def set_graph_fusion(level) -> int:
 
 
 # kinds of graph_kernel_times: the loops a stage kernel runs besides
-# res_calc and update, and its res traffic
+# res_calc and update, its res traffic, and the form of its adt_calc
 KIND_BRES, KIND_SAVE, KIND_ADT, KIND_READS_RES, KIND_STORES_RES = 1, 2, 4, 8, 16
+KIND_ADT_SIDES = 32
 
 
 def set_graph_kernel_timing(every: int) -> int:

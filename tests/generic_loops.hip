@@ -97,7 +97,25 @@ __device__ inline void gen_res_calc(const double *x1, const double *x2, const do
   res2[3] -= f;
 }
 
+// the owner-tile form: every written argument an OP_INC through the edge ->
+// node map (a float dim-3 dat and an int count, each through both indices)
+// or an OP_GBL OP_MAX; the rest read only
+__device__ inline void gen_edge_owner(const double *x1, const double *x2, float *n1, float *n2,
+                                      int *c1, int *c2, const float *w, double *longest) {
+  const double dx = x2[0] - x1[0], dy = x2[1] - x1[1];
+  n1[0] += *w;
+  n1[1] += (float)dx;
+  n1[2] += 1.0f;
+  n2[0] -= *w;
+  n2[1] -= (float)dy;
+  n2[2] += 1.0f;
+  *c1 += 1;
+  *c2 += 2;
+  *longest = MAX(*longest, sqrt(dx * dx + dy * dy));
+}
+
 OP2_DEVICE_LOOP(gen_edge_sum)
+OP2_DEVICE_LOOP(gen_edge_owner)
 OP2_DEVICE_LOOP(gen_res_calc)
 
 // airfoil update's signature (airfoil/update.h) with another body: half the

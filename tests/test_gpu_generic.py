@@ -235,6 +235,101 @@ def test_generic_res_calc_equals_the_device_kernel(runtime, renumber):
     assert np.max(np.abs(got - native)) <= 1e-13 * np.max(np.abs(native))
 
 
+def test_generic_res_calc_on_owner_tiles_is_the_sequential_sum(runtime):
+    """gen_res_calc has the owner-tile form (its written arguments are the
+    two OP_INC res through pecell): each cell adds its edges' increments in
+    edge order, so on the mesh's own numbering res is bit for bit the
+    oracle's sequential res_calc (common/op_seq.h order)."""
+    note("generic res_calc on owner tiles vs the sequential oracle, 60x30 random")
+    from oplus2_matlab_amd import airfoil as AF
+    from oracle import oracle as O
+    mesh = M.o_mesh(60, 30, "random", seed=6)
+    c = M.flow_constants().as_array()
+    rng = np.random.Generator(np.random.PCG64(13))
+    q0 = np.tile(c[6:10], (mesh.ncell, 1)) * (1 + 0.02 * rng.standard_normal((mesh.ncell, 4)))
+    a = AF.Airfoil(mesh, c, "double", q0=q0)
+    a.adt_calc()
+    adt = op2.op_fetch_data(a.p_adt).copy()
+    res0 = 1e-3 * rng.standard_normal(4 * mesh.ncell)
+    pres = op2.op_decl_dat(a.cells, 4, "double", res0.copy(), "pres")
+    args = list(a.res_calc_args())
+    args[6] = op_arg_dat(pres, 0, a.pecell, 4, "double", OP_INC)
+    args[7] = op_arg_dat(pres, 1, a.pecell, 4, "double", OP_INC)
+    arr = (op2.OpArg * 8)(*args)
+    info = (C.c_char * 128)()  # op2_generic_owner_plan
+    arg_info = (C.c_int * 16)()
+    assert op2.lib().op2_generic_owner_plan_get(b"gen_res_calc", a.edges._p, 8, arr, 0, info,
+                                                arg_info) == 0
+    # OP_INC pairs of res1, res2; target-row places of (pedge 0, 1), (pecell 0, 1)
+    assert list(arg_info) == [-1, -1, -1, -1, -1, -1, 0, 1] + [0, 1, 2, 3, 2, 3, 2, 3]
+    op2.op_par_loop(None, "gen_res_calc", a.edges, *args)
+    want = res0.reshape(-1, 4).copy()
+    O.res_calc("f64", mesh, np.ascontiguousarray(mesh.x), np.ascontiguousarray(q0), adt, want, c)
+    got = op2.op_fetch_data(pres).reshape(-1, 4)
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("renumber,part", [(False, 64), (False, 256), (True, 128)])
+def test_owner_tile_increments_in_element_order(runtime, renumber, part):
+    """gen_edge_owner on owner tiles: float dim-3 and int increments through
+    both indices of the edge -> node map (two dats, two arguments each) and
+    an OP_GBL OP_MAX; each node's float sum is the sequential one (element
+    order, bit for bit on the mesh's own numbering), the counts and the
+    maximum exact.  A loop that also writes a direct dat (gen_edge) or
+    increments through two maps keeps the coloured plan."""
+    note(f"generic owner tiles: gen_edge_owner, renumber={renumber}, part={part}")
+    mesh = M.o_mesh(90, 45, "random", seed=21)
+    rng = np.random.Generator(np.random.PCG64(8))
+    nodes = op2.op_decl_set(mesh.nnode, "nodes")
+    edges = op2.op_decl_set(mesh.nedge, "edges")
+    pedge = op2.op_decl_map(edges, nodes, 2, mesh.edge, "pedge")
+    x = np.ascontiguousarray(mesh.x)
+    nd = np.ascontiguousarray(rng.random((mesh.nnode, 3)).astype(np.float32))
+    cnt = np.zeros(mesh.nnode, np.int32)
+    w = np.ascontiguousarray(rng.random(mesh.nedge).astype(np.float32))
+    px = op2.op_decl_dat(nodes, 2, "double", x, "px")
+    pn = op2.op_decl_dat(nodes, 3, "float", nd.copy(), "pn")
+    pc = op2.op_decl_dat(nodes, 1, "int", cnt.copy(), "pc")
+    pw = op2.op_decl_dat(edges, 1, "float", w, "pw")
+    if renumber:
+        op2.renumber(pedge)
+    op2.set_loop_config("gen_edge_owner", 0, part, 0)
+    longest = np.zeros(1)
+    args = (op_arg_dat(px, 0, pedge, 2, "double", OP_READ),
+            op_arg_dat(px, 1, pedge, 2, "double", OP_READ),
+            op_arg_dat(pn, 0, pedge, 3, "float", OP_INC),
+            op_arg_dat(pn, 1, pedge, 3, "float", OP_INC),
+            op_arg_dat(pc, 0, pedge, 1, "int", OP_INC),
+            op_arg_dat(pc, 1, pedge, 1, "int", OP_INC),
+            op_arg_dat(pw, -1, OP_ID, 1, "float", OP_READ),
+            op_arg_gbl(longest, 1, "double", OP_MAX))
+    for _ in range(2):
+        op2.op_par_loop(None, "gen_edge_owner", edges, *args)
+    e1, e2 = mesh.edge[:, 0], mesh.edge[:, 1]
+    dx = x[e2, 0] - x[e1, 0]
+    dy = x[e2, 1] - x[e1, 1]
+    # the sequential order: edge by edge, node 1's increment then node 2's,
+    # each rounded to float
+    idx = np.stack([e1, e2], axis=1).reshape(-1)
+    want = nd.copy()
+    for _ in range(2):
+        for comp, v1, v2 in ((0, w, -w), (1, dx.astype(np.float32), -dy.astype(np.float32)),
+                             (2, np.ones_like(w), np.ones_like(w))):
+            vals = np.stack([v1, v2], axis=1).reshape(-1).astype(np.float32)
+            col = want[:, comp].copy()
+            np.add.at(col, idx, vals)
+            want[:, comp] = col
+    got = op2.op_fetch_data(pn).reshape(-1, 3)
+    if renumber:
+        np.testing.assert_allclose(got, want, rtol=0, atol=2e-5)
+    else:
+        assert np.array_equal(got, want)
+    want_cnt = 2 * (np.bincount(e1, minlength=mesh.nnode) + 2 * np.bincount(e2, minlength=mesh.nnode))
+    assert np.array_equal(op2.op_fetch_data(pc), want_cnt.astype(np.int32))
+    np.testing.assert_allclose(longest[0], np.sqrt(dx * dx + dy * dy).max(), rtol=1e-15)
+    op2.set_loop_config("gen_edge_owner", 0, 0, 0)
+
+
 def test_reductions_in_a_captured_graph(runtime):
     """With deferred reductions a generic loop's OP_GBL results go to the
     device log (the loop's sum for OP_INC, the result including the caller's
