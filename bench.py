@@ -116,23 +116,24 @@ TRAFFIC_PROFILE = os.path.join("profiles", "airfoil_2400x1200_fp64_traffic.md")
 def measured_traffic(ni, nj, res_cfg, keys):
     """Mean HBM bytes per launch over `keys` (rows of the committed rocprofv3
     counter summary: FETCH_SIZE x copy-kernel factor + WRITE_SIZE, in MB),
-    when this run is the profiled configuration (2400x1200 FP64, cartesian
+    when this run is a profiled configuration (2400x1200 FP64, cartesian
     ordering renumbered, and the res_calc strategy, part and block size on
-    the summary's `config:` line); else None."""
+    one of the summary's `config:` lines); else None."""
     path = os.path.join(ROOT, TRAFFIC_PROFILE)
     if (ni, nj) != (2400, 1200) or not res_cfg or not os.path.exists(path):
         return None
+    # keyed rows follow the `config:` line of the runs they come from
     rows, cfg = {}, None
     for line in open(path):
         if line.startswith("config:"):
             cfg = tuple(int(v) for v in line.split()[1:4])
         elif line.startswith("| key:"):
             f = [c.strip() for c in line.split("|")]
-            rows[f[1][4:].strip().strip("`")] = float(f[-4]) * 1e6
+            rows[(cfg, f[1][4:].strip().strip("`"))] = float(f[-4]) * 1e6
     got = (res_cfg.get("strategy"), res_cfg.get("part_size"), res_cfg.get("block_size"))
-    if cfg != got or not keys or any(k not in rows for k in keys):
+    if not keys or any((got, k) not in rows for k in keys):
         return None
-    return sum(rows[k] for k in keys) / len(keys)
+    return sum(rows[(got, k)] for k in keys) / len(keys)
 
 
 def capture_graphs(af, graph_iters, nsteps):
@@ -227,16 +228,20 @@ def stage_bytes(mesh, kind, size):
     direction.  Always: pedge and pecell (8 B per edge each), x (2 reals per
     node), q, adt and qold read and the new q and adt written (4+1+4+4+1
     reals per cell), and for bres_calc pbedge, pbecell and bound (16 B per
-    boundary edge); SAVE writes qold (4 reals per cell), ADT reads pcell (16
+    boundary edge); SAVE writes qold (4 reals per cell; none when the new q
+    is written over qold in place of its own store; a stage whose qold is
+    that q reads it once), ADT reads pcell (16
     B per cell) or, in the side form, the side recipes (2 B per cell), and
     res (4 reals per cell) is read and/or stored as the kind says.  The rms
     partials (8 B per tile) are left out."""
     from oplus2_matlab_amd import op2
 
     b = 16 * mesh.nedge + 2 * size * mesh.nnode + 14 * size * mesh.ncell
+    if kind & op2.KIND_QOLD_IS_Q:  # qold and q are the same record: read once
+        b -= 4 * size * mesh.ncell
     if kind & op2.KIND_BRES:
         b += 16 * mesh.nbedge
-    if kind & op2.KIND_SAVE:
+    if kind & op2.KIND_SAVE and not kind & op2.KIND_SAVE_IN_PLACE:
         b += 4 * size * mesh.ncell
     if kind & op2.KIND_ADT:
         b += (2 if kind & op2.KIND_ADT_SIDES else 16) * mesh.ncell
@@ -444,7 +449,9 @@ def main():
     stage = stage_roofline(mesh, r64["stages"], 8)
     if stage:
         traffic = measured_traffic(ni, nj, res_cfg,
-                                   [f"stage/{kd & 7}" + ("s" if kd & op2.KIND_ADT_SIDES else "")
+                                   [f"stage/{kd & 7}" + ("s" if kd & op2.KIND_ADT_SIDES else "") +
+                                    ("i" if kd & op2.KIND_SAVE_IN_PLACE else "") +
+                                    ("q" if kd & op2.KIND_QOLD_IS_Q else "")
                                     for kd, _ in r64["stages"]]) \
             if profiled else None
         roofline = {"bound": "hbm", "achieved": stage["GBs"], "peak": HBM_PEAK_GBS,

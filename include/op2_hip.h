@@ -411,7 +411,9 @@ int op2_set_graph_fusion(int level);
  * save_soln, 4 the adt_calc that follows, 8 it reads res (else res is taken
  * as zero), 16 it stores res (zeros), 32 its adt_calc takes the cells' sides
  * from the tile's edges and a 2-byte recipe per cell (else it reads pcell
- * and the nodes).  op2_set_graph_kernel_timing returns
+ * and the nodes), 64 its save_soln is its q store (the new q written over
+ * qold), 128 the qold it reads is its q (left there by such a save_soln).
+ * op2_set_graph_kernel_timing returns
  * the previous setting, or -1 during a capture (no change).  No counterpart
  * in the reference (its OP_kernels records time whole loops outside
  * graphs). */

@@ -1993,15 +1993,18 @@ const int *tiles_of_meshes(const StageBufs &b, op_map m) {
   return nullptr;
 }
 
-// The second buffer of every dat a stage writes (q, adt), and which of the
-// two holds the dat's current values during a capture (true: the second).
+// The second buffer of every dat a stage writes (q, adt), and, during a
+// capture, the buffer that holds each such dat's current values: null for
+// its own; else its second buffer or, after a stage that also ran the next
+// save_soln, the buffer of that save_soln's target (qold), which then holds
+// the same values.
 std::map<op_dat, void *> &shadow_bufs() {
   static std::map<op_dat, void *> m;
   return m;
 }
 
-std::map<op_dat, bool> &in_shadow() {
-  static std::map<op_dat, bool> m;
+std::map<op_dat, char *> &live_buf() {
+  static std::map<op_dat, char *> m;
   return m;
 }
 
@@ -2101,7 +2104,7 @@ void release_exec_caches() {
   stage_bufs().clear();
   for (auto &s : shadow_bufs()) dev_free(s.second);
   shadow_bufs().clear();
-  in_shadow().clear();
+  live_buf().clear();
 }
 
 void forget_plan_caches(const Plan *p) {
@@ -2120,7 +2123,7 @@ void forget_plan_caches(const Plan *p) {
 // batched update's cell -> mesh map.
 void prepare_capture() {
   Runtime &r = rt();
-  for (auto &kv : in_shadow()) kv.second = false;
+  for (auto &kv : live_buf()) kv.second = nullptr;
   if (r.graph_fusion < 2) return;
   for (auto &up : r.plans) {
     const Plan *p = up.get();
@@ -2145,7 +2148,7 @@ void prepare_capture() {
       op_dat d = p->dats[a];
       if (!shadow_bufs().count(d))
         shadow_bufs()[d] = dev_alloc((size_t)d->size * d->set->size);
-      in_shadow()[d] = false;
+      live_buf()[d] = nullptr;
     }
     for (auto &sp : seg_cache()) {
       bool known = false;
@@ -2674,12 +2677,13 @@ Stage &stage() {
 }
 
 char *live_ptr(op_dat d) {
-  auto it = in_shadow().find(d);
-  return it != in_shadow().end() && it->second ? (char *)shadow_bufs()[d] : d->data_d;
+  auto it = live_buf().find(d);
+  return it != live_buf().end() && it->second ? it->second : d->data_d;
 }
 
+// a buffer of d that does not hold its current values
 char *other_ptr(op_dat d) {
-  return in_shadow()[d] ? d->data_d : (char *)shadow_bufs().at(d);
+  return live_ptr(d) == d->data_d ? (char *)shadow_bufs().at(d) : d->data_d;
 }
 
 template <typename T>
@@ -2695,15 +2699,14 @@ __global__ __launch_bounds__(kFixedBlock) void k_copy_elems(const T *__restrict_
 // own (before a loop other than a stage reads or writes dats, and at the end
 // of a capture).
 void sync_shadows() {
-  for (auto &kv : in_shadow()) {
+  for (auto &kv : live_buf()) {
     if (!kv.second) continue;
     op_dat d = kv.first;
     const long long words = (long long)d->size * d->set->size / 4;
     hipLaunchKernelGGL((k_copy_elems<unsigned>), dim3(grid_for(words, 1 << 16)), dim3(kFixedBlock), 0,
-                       rt().stream, (const unsigned *)shadow_bufs()[d], (unsigned *)d->data_d,
-                       words);
+                       rt().stream, (const unsigned *)kv.second, (unsigned *)d->data_d, words);
     check_launch("graph stage");
-    kv.second = false;
+    kv.second = nullptr;
   }
 }
 
@@ -2797,9 +2800,17 @@ void run_stage(Stage &st, bool store_res) {
   a.adt_in = (const T *)live_ptr(adt);
   a.qold = (const T *)live_ptr(st.upd[0].dat);
   a.res = (T *)res->data_d;
-  a.q_out = (T *)other_ptr(q);
+  // with the next save_soln joined, the new q is the next iteration's qold:
+  // it is written once, over the qold the update reads (each cell's own,
+  // read by its own thread before the store), and q then lives in qold's
+  // buffer; not when q already lives there (other tiles read it as halo)
+  const op_dat qold = st.upd[0].dat;
+  const bool save_in_place = save && st.save[1].dat == qold && qold != q && qold != adt &&
+                             !shadow_bufs().count(qold) && live_ptr(q) != qold->data_d;
+  char *const q_dst = save_in_place ? qold->data_d : other_ptr(q);
+  a.q_out = (T *)q_dst;
   a.adt_out = (T *)other_ptr(adt);
-  a.qold_out = save ? (T *)st.save[1].dat->data_d : nullptr;
+  a.qold_out = save && !save_in_place ? (T *)st.save[1].dat->data_d : nullptr;
   a.pcell = has_adt ? (const int4 *)st.adt[0].map->map_d : nullptr;
   a.xc = has_adt ? (const T *)live_ptr(st.adt[0].dat) : nullptr;
   const int par = sb->parity;
@@ -2828,18 +2839,20 @@ void run_stage(Stage &st, bool store_res) {
         adt_mode = kAdtSides;
       }
   }
-  auto kern = stage_kernel<NT, T>(own, p->owned_identity, has_bres, save, adt_mode);
+  auto kern = stage_kernel<NT, T>(own, p->owned_identity, has_bres, save && !save_in_place,
+                                  adt_mode);
   const size_t lds = std::max(sizeof(T) * 4 * (size_t)p->max_positions, (size_t)64);
   allow_lds(kern, lds);
   const CarriedFold fold = take_fold(p->nblocks);
   const int mark = mark_begin(has_bres * 1 + save * 2 + has_adt * 4 + !st.zero_in * 8 +
-                              store_res * 16 + (adt_mode == kAdtSides) * 32);
+                              store_res * 16 + (adt_mode == kAdtSides) * 32 +
+                              save_in_place * 64 + ((char *)a.qold == (char *)a.q_in) * 128);
   hipLaunchKernelGGL(kern, dim3(p->nblocks), dim3(NT), lds, r.stream, a, bi, airfoil_consts<T>(),
                      fold, st.zero_in, store_res);
   check_launch("graph stage");
   mark_end(mark);
-  in_shadow()[q] = !in_shadow()[q];
-  in_shadow()[adt] = !in_shadow()[adt];
+  live_buf()[q] = q_dst == q->data_d ? nullptr : q_dst;
+  live_buf()[adt] = (char *)a.adt_out == adt->data_d ? nullptr : (char *)a.adt_out;
   CarriedFold f{};
   f.partials = sb->partials[par];
   f.log = r.deferred ? r.d_log : nullptr;
@@ -2910,7 +2923,7 @@ void fusion_flush() {
 }
 void fusion_drop() {
   stage().active = false;
-  for (auto &kv : in_shadow()) kv.second = false;
+  for (auto &kv : live_buf()) kv.second = nullptr;
   held().active = false;
   held().save_args.clear();
   held().bres_args.clear();

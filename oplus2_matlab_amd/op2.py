@@ -516,7 +516,7 @@ def set_graph_fusion(level) -> int:
 # kinds of graph_kernel_times: the loops a stage kernel runs besides
 # res_calc and update, its res traffic, and the form of its adt_calc
 KIND_BRES, KIND_SAVE, KIND_ADT, KIND_READS_RES, KIND_STORES_RES = 1, 2, 4, 8, 16
-KIND_ADT_SIDES = 32
+KIND_ADT_SIDES, KIND_SAVE_IN_PLACE, KIND_QOLD_IS_Q = 32, 64, 128
 
 
 def set_graph_kernel_timing(every: int) -> int:

@@ -427,10 +427,13 @@ def test_stage_kernel_times_inside_graphs(runtime, dtype):
         op2.graph_launch(g, 2)
     times = op2.graph_kernel_times(every)
     note(f"stage kernels {times}")
-    # adt_calc from the tiles' edges (128-cell tiles fit the kernel's passes)
-    B, S, A, RR, SR = (op2.KIND_BRES, op2.KIND_SAVE, op2.KIND_ADT | op2.KIND_ADT_SIDES,
-                       op2.KIND_READS_RES, op2.KIND_STORES_RES)
-    kinds = [B | A | RR, B | S | A, B | A, B | S | A, B | A, B | SR]
+    # adt_calc from the tiles' edges (128-cell tiles fit the kernel's passes);
+    # the new q written over qold when save_soln joins, so that the next
+    # stage reads its qold and its q from one buffer
+    B, S, A, RR, SR, Q = (op2.KIND_BRES, op2.KIND_SAVE | op2.KIND_SAVE_IN_PLACE,
+                          op2.KIND_ADT | op2.KIND_ADT_SIDES, op2.KIND_READS_RES,
+                          op2.KIND_STORES_RES, op2.KIND_QOLD_IS_Q)
+    kinds = [B | A | RR, B | S | A, B | A | Q, B | S | A, B | A | Q, B | SR]
     assert [k for k, _ in times] == kinds
     assert all(0 < ms < 1000 for _, ms in times)
     times2 = op2.graph_kernel_times(second)
@@ -439,3 +442,47 @@ def test_stage_kernel_times_inside_graphs(runtime, dtype):
     assert op2.graph_kernel_times(plain) == []
     op2.reduction_log()
     op2.set_deferred_reductions(False)
+
+
+@pytest.mark.parametrize("dtype", ["double", "float"])
+@pytest.mark.parametrize("tail", ["iteration", "save_soln"])
+def test_save_soln_joined_in_place_keeps_every_dat(runtime, dtype, tail):
+    """With the next save_soln joined to a stage the new q is stored once,
+    over qold, and q lives in qold's buffer until the next stage: after the
+    graph q, qold, adt and res hold the unfused graph's bits, whether the
+    capture ends with a whole iteration (qold = the last iteration's start)
+    or right after a save_soln (qold = q)."""
+    note(f"save_soln in place: graphs of 3 iterations ending with {tail}, {dtype}")
+    mesh = M.o_mesh(40, 20, "random", seed=17)
+    c = M.flow_constants().as_array()
+    out = []
+    for i, fusion in enumerate((op2.FUSION_STAGES, op2.FUSION_OFF)):
+        if i:
+            op2.op_init([], 0)
+        for loop in ("res_calc", "bres_calc"):
+            op2.set_loop_config(loop, op2.STRATEGY_OWNER_GATHER, 128, 0)
+        op2.set_graph_fusion(fusion)
+        af = AF.Airfoil(mesh, c, dtype, renumber=True)
+        af.iterate()
+        op2.set_deferred_reductions(True)
+        op2.graph_begin()
+        for _ in range(3):
+            af.save_soln()
+            for _ in range(2):
+                af.adt_calc()
+                af.res_calc()
+                af.bres_calc()
+                af.update()
+        if tail == "save_soln":
+            af.save_soln()
+        gid = op2.graph_end()
+        op2.graph_launch(gid, 3)
+        op2.reduction_log()
+        op2.set_deferred_reductions(False)
+        out.append([v.copy() for v in af.fetch()])
+        op2.op_exit()
+    op2.op_init([], 0)
+    for got, want in zip(out[0], out[1]):
+        assert np.array_equal(got, want)
+    if tail == "save_soln":
+        assert np.array_equal(out[0][0], out[0][1])
