@@ -99,6 +99,9 @@ def parse():
                     help="meshes of the batched measurement (0 = skip)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="target CPU time of the cpu_baseline sample (0 = skip)")
+    ap.add_argument("--stage-tiles", default="0/0",
+                    help="PART/BLOCK of the stage kernels' tiles (0/0 = the runtime's default, "
+                         "256/256; -1/0 = res_calc's configuration)")
     ap.add_argument("--no-fusion", dest="fusion", action="store_false",
                     help="capture the graphs without loop fusion")
     ap.add_argument("--graph-iters", type=int, default=50,
@@ -113,14 +116,15 @@ def parse():
 TRAFFIC_PROFILE = os.path.join("profiles", "airfoil_2400x1200_fp64_traffic.md")
 
 
-def measured_traffic(ni, nj, res_cfg, keys):
+def measured_traffic(ni, nj, cfg, keys):
     """Mean HBM bytes per launch over `keys` (rows of the committed rocprofv3
     counter summary: FETCH_SIZE x copy-kernel factor + WRITE_SIZE, in MB),
     when this run is a profiled configuration (2400x1200 FP64, cartesian
-    ordering renumbered, and the res_calc strategy, part and block size on
-    one of the summary's `config:` lines); else None."""
+    ordering renumbered, and `cfg`, the strategy, part and block size of the
+    tiles the keyed kernels ran on, on one of the summary's `config:`
+    lines); else None."""
     path = os.path.join(ROOT, TRAFFIC_PROFILE)
-    if (ni, nj) != (2400, 1200) or not res_cfg or not os.path.exists(path):
+    if (ni, nj) != (2400, 1200) or not cfg or not os.path.exists(path):
         return None
     # keyed rows follow the `config:` line of the runs they come from
     rows, cfg = {}, None
@@ -130,7 +134,7 @@ def measured_traffic(ni, nj, res_cfg, keys):
         elif line.startswith("| key:"):
             f = [c.strip() for c in line.split("|")]
             rows[(cfg, f[1][4:].strip().strip("`"))] = float(f[-4]) * 1e6
-    got = (res_cfg.get("strategy"), res_cfg.get("part_size"), res_cfg.get("block_size"))
+    got = (cfg.get("strategy"), cfg.get("part_size"), cfg.get("block_size"))
     if not keys or any((got, k) not in rows for k in keys):
         return None
     return sum(rows[(got, k)] for k in keys) / len(keys)
@@ -430,6 +434,8 @@ def main():
     from oplus2_matlab_amd import op2
 
     op2.set_graph_fusion(a.fusion)
+    st_part, st_block = (int(v) for v in a.stage_tiles.split("/"))
+    op2.set_stage_config(st_part, st_block)
     ni, nj = (int(v) for v in a.mesh.lower().split("x"))
     mesh = M.o_mesh(ni, nj, a.ordering, seed=1)
     consts = M.flow_constants().as_array()
@@ -447,8 +453,11 @@ def main():
                 "bytes_per_edge": rc["bytes_per_call"] / mesh.nedge,
                 "us_per_launch": rc["us_per_call"]}
     stage = stage_roofline(mesh, r64["stages"], 8)
+    stage_cfg = dict(res_cfg or {}) if st_part < 0 else \
+        {"strategy": STRATEGIES["owner-gather"], "part_size": st_part or 256,
+         "block_size": (st_block or 256) if st_part else 256}
     if stage:
-        traffic = measured_traffic(ni, nj, res_cfg,
+        traffic = measured_traffic(ni, nj, stage_cfg,
                                    [f"stage/{kd & 7}" + ("s" if kd & op2.KIND_ADT_SIDES else "") +
                                     ("i" if kd & op2.KIND_SAVE_IN_PLACE else "") +
                                     ("q" if kd & op2.KIND_QOLD_IS_Q else "")
@@ -484,6 +493,7 @@ def main():
                    "strategy": a.strategy, "part_size": a.part or None,
                    "renumbered": a.renumber == "on", "graph_fusion": a.fusion,
                    "graph_iters": r64["graph_iters"],
+                   "stage_tiles": {k: stage_cfg.get(k) for k in ("part_size", "block_size")},
                    "tuned": r64["tuned"]},
         "roofline": roofline,
         "res_calc_roofline": res_roof,

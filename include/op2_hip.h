@@ -399,6 +399,15 @@ int op2_graph_destroy(int id);
  * previous level, or -1 during a capture (no change). */
 int op2_set_graph_fusion(int level);
 
+/* Tiles of the level-2 stage kernels: cells per tile and threads per
+ * workgroup (128, 256 or 512; part_size at most twice the threads).  A
+ * stage tile's plan is built at op2_graph_begin from the res_calc that ran
+ * on owner-gather tiles before it, with this part size; res_calc's own
+ * launches keep their configuration.  part_size 0: 256 cells on 256
+ * threads (the default); -1: res_calc's configuration.  Returns 0, or -1 on
+ * an invalid pair or during a capture (no change). */
+int op2_set_stage_config(int part_size, int block_size);
+
 /* Kernel timing inside graphs.  With every = n > 0, the stage kernels
  * (fusion level 2) of the captures that follow are counted from 1 in each
  * capture and those numbered n, 2n, 3n, ... are bracketed by two

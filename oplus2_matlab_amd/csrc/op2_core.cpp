@@ -844,6 +844,25 @@ int op2_set_graph_fusion(int level) {
   return prev;
 }
 
+int op2_set_stage_config(int part_size, int block_size) {
+  Runtime &r = rt();
+  if (r.capturing) return -1;
+  if (part_size < 0) {
+    r.stage_part = -1;
+    r.stage_block = 0;
+    return 0;
+  }
+  if (part_size == 0) {
+    r.stage_part = r.stage_block = 0;
+    return 0;
+  }
+  const int block = block_size ? block_size : 256;
+  if ((block != 128 && block != 256 && block != 512) || part_size > 2 * block) return -1;
+  r.stage_part = part_size;
+  r.stage_block = block;
+  return 0;
+}
+
 int op2_graph_launch(int id, int count) {
   API_BEGIN
   Runtime &r = rt();

@@ -1426,6 +1426,21 @@ int part_of(const std::string &loop, int dflt) {
   return p ? p : dflt;
 }
 
+// Tiles of the fused stage kernels (op2_set_stage_config).  By default 256
+// cells on 256 threads whatever res_calc's own configuration: a stage tile
+// runs flux, update and adt_calc phases, and more, smaller workgroups per CU
+// overlap those phases better (2.88M cells: 256/256 beat res_calc's tuned
+// 512/512 by 5% in FP64 and 4% in the FP32 form; 128-cell tiles exceed the
+// side form of adt_calc).
+int stage_part_of(const std::string &loop) {
+  const Runtime &r = rt();
+  return r.stage_part < 0 ? part_of(loop, 256) : r.stage_part ? r.stage_part : 256;
+}
+int stage_block_of(const std::string &loop) {
+  const Runtime &r = rt();
+  return r.stage_part < 0 ? block_of(loop) : r.stage_block ? r.stage_block : 256;
+}
+
 Plan *plan_for(const char *name, op_set set, int nargs, op_arg *args, int strategy, int part) {
   PlanRequest req{name, set, nargs, args, strategy, part};
   Plan *p = get_plan(req);
@@ -2125,10 +2140,23 @@ void prepare_capture() {
   Runtime &r = rt();
   for (auto &kv : live_buf()) kv.second = nullptr;
   if (r.graph_fusion < 2) return;
+  // the stage tiles of every res_calc that ran on gather tiles, built before
+  // the capture (no allocation inside it)
+  std::vector<Plan *> res_plans;
+  for (auto &up : r.plans)
+    if (up->airfoil_res && up->strategy == OP2_STRATEGY_OWNER_GATHER)
+      res_plans.push_back(up.get());
+  for (Plan *p : res_plans) {
+    const int part = stage_part_of(p->loop);
+    if (part == p->part_size) continue;
+    Plan *sp = plan_for(p->loop.c_str(), p->set, (int)p->args.size(), p->args.data(),
+                        OP2_STRATEGY_OWNER_GATHER, part);
+    sp->airfoil_res = true;
+  }
   for (auto &up : r.plans) {
     const Plan *p = up.get();
     if (!p->airfoil_res || p->strategy != OP2_STRATEGY_OWNER_GATHER || !p->d_ell ||
-        p->max_owned > 2 * block_of(p->loop))
+        p->max_owned > 2 * std::max(block_of(p->loop), stage_block_of(p->loop)))
       continue;
     const op_set cells = p->groups[p->owner_group].map->to;
     if ((int)p->owned.size() != cells->size) continue;
@@ -2717,10 +2745,11 @@ Plan *starts_stage(const char *name, Loop l, int rb, op_set set, int nargs, op_a
   auto ot = r.online.find(name);
   if (ot != r.online.end() && !ot->second.done) return nullptr;
   if (strategy_of(name) != OP2_STRATEGY_OWNER_GATHER) return nullptr;
-  Plan *p = plan_for(name, set, nargs, a, OP2_STRATEGY_OWNER_GATHER, part_of(name, 256));
-  if (!bufs_of(p) || !shadow_bufs().count(a[2].dat) || !shadow_bufs().count(a[4].dat))
+  // (built by prepare_capture: nothing is built inside a capture)
+  Plan *p = find_plan({name, set, nargs, a, OP2_STRATEGY_OWNER_GATHER, stage_part_of(name)});
+  if (!p || !bufs_of(p) || !shadow_bufs().count(a[2].dat) || !shadow_bufs().count(a[4].dat))
     return nullptr;
-  if (p->max_owned > 2 * block_of(name)) return nullptr;  // at most two cells per thread
+  if (p->max_owned > 2 * stage_block_of(name)) return nullptr;  // at most two cells per thread
   const op_dat q = a[2].dat, adt = a[4].dat, res = a[6].dat;
   if (q == adt || q == res || adt == res || a[0].dat == q || a[0].dat == adt) return nullptr;
   (void)rb;
@@ -2903,8 +2932,7 @@ void launch_stage(bool next_takes_res) {
     }
     return;
   }
-  // the stage kernel runs with res_calc's block size
-  with_block(block_of(st.name), [&](auto B) {
+  with_block(stage_block_of(st.name), [&](auto B) {
     constexpr int BS = decltype(B)::value;
     if (st.rb == 8)
       run_stage<BS, double>(st, !next_takes_res);

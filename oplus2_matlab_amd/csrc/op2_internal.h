@@ -67,6 +67,7 @@ struct Plan {
   std::vector<int> idxs;
   std::vector<int> accs;
   std::vector<op_dat> dats;
+  std::vector<op_arg> args;  // the request's arguments (a plan of another part size)
 
   // layout
   int nblocks = 0, max_block = 0;
@@ -206,6 +207,9 @@ struct Runtime {
   // graph_fusion, adjacent loops that can share a pass are fused there
   bool capturing = false;
   int graph_fusion = 2;  // 0 off, 1 loop fusion, 2 also one kernel per Runge-Kutta stage
+  // tiles of the fused stage kernels (op2_set_stage_config): cells per tile
+  // and threads; 0 = 256 and 256, -1 = res_calc's own configuration
+  int stage_part = 0, stage_block = 0;
   std::vector<hipGraphExec_t> graphs;
   std::vector<void *> retired;  // replaced scratch buffers that graphs may use
   std::map<std::string, OnlineTune> online;
@@ -294,6 +298,7 @@ struct PlanRequest {
   int part_size;
 };
 Plan *get_plan(const PlanRequest &req);
+Plan *find_plan(const PlanRequest &req);  // null if not built (builds nothing)
 long long verify_plan(const Plan &p);
 // frees the op_plan_core views and the OP_plans array (op_exit)
 void release_ref_plans();

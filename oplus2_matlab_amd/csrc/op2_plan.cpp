@@ -309,6 +309,7 @@ static Plan *build(const PlanRequest &req) {
     P.accs.push_back((int)req.args[a].acc);
     P.dats.push_back(req.args[a].dat);
   }
+  P.args.assign(req.args, req.args + req.nargs);
 
   // groups (distinct maps) and slots (distinct (map, idx))
   const bool generic = req.strategy == kStrategyGeneric;
@@ -687,6 +688,12 @@ void drop_plan(Plan *p) {
       r.plans.erase(r.plans.begin() + i);
       return;
     }
+}
+
+Plan *find_plan(const PlanRequest &req) {
+  for (auto &p : rt().plans)
+    if (same_key(*p, req)) return p.get();
+  return nullptr;
 }
 
 Plan *get_plan(const PlanRequest &req) {

@@ -156,6 +156,7 @@ SIGNATURES = {
     "op2_graph_launch": (C.c_int, [C.c_int, C.c_int]),
     "op2_graph_destroy": (C.c_int, [C.c_int]),
     "op2_set_graph_fusion": (C.c_int, [C.c_int]),
+    "op2_set_stage_config": (C.c_int, [C.c_int, C.c_int]),
     "op2_set_graph_kernel_timing": (C.c_int, [C.c_int]),
     "op2_graph_kernel_times": (C.c_int, [C.c_int, C.c_int, _P, _P]),
     "op2_reduction_log_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
@@ -511,6 +512,17 @@ def set_graph_fusion(level) -> int:
     if prev < 0:
         raise OP2Error("op2_set_graph_fusion: not during graph capture")
     return prev
+
+
+def set_stage_config(part_size: int = 0, block_size: int = 0) -> None:
+    """Tiles of the stage kernels (FUSION_STAGES): cells per tile and
+    threads (128, 256 or 512, part_size at most twice that).  0: 256 cells
+    on 256 threads (the default); -1: res_calc's own configuration.  The
+    stage plan is built at graph_begin; res_calc's own launches keep their
+    configuration."""
+    if _call("op2_set_stage_config", int(part_size), int(block_size)) != 0:
+        raise OP2Error(f"op2_set_stage_config: invalid tiles {part_size}/{block_size} "
+                       "or graph capture active")
 
 
 # kinds of graph_kernel_times: the loops a stage kernel runs besides

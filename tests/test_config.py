@@ -48,3 +48,17 @@ def test_op_block_size_from_argv(host_runtime):
     L.op2_clear_error()
     L.op_init_core(2, argv, 0)
     assert "not supported" in L.op2_last_error().decode()
+
+
+@pytest.mark.parametrize("part,block", [(600, 256), (100, 64), (256, 384), (1100, 512)])
+def test_invalid_stage_tiles_are_refused(host_runtime, part, block):
+    """Stage tiles need a supported block size and at most two cells per
+    thread; the setting is left unchanged."""
+    with pytest.raises(op2.OP2Error, match="invalid tiles"):
+        op2.set_stage_config(part, block)
+
+
+def test_stage_tiles_accept_defaults_and_res_calc(host_runtime):
+    for part, block in [(0, 0), (-1, 0), (512, 256), (128, 128), (1024, 512), (300, 0)]:
+        op2.set_stage_config(part, block)
+    op2.set_stage_config(0)

@@ -31,6 +31,7 @@ def runtime():
         op2.set_loop_config(loop, op2.STRATEGY_OWNER_GATHER, 128, 0)
     yield
     op2.set_graph_fusion(True)
+    op2.set_stage_config(0)
     op2.set_deferred_reductions(False)
     op2.op_exit()
     op2.set_loop_config(None, 0, 0, 0)
@@ -336,6 +337,7 @@ def test_fused_graphs_at_other_block_sizes(runtime, block, dtype):
 
     def run(level):
         op2.set_loop_config(None, 0, 0, block)
+        op2.set_stage_config(-1)  # stage tiles as res_calc's
         for loop in ("res_calc", "bres_calc"):
             op2.set_loop_config(loop, op2.STRATEGY_OWNER_GATHER, 256, 0)
         return graph_run(mesh, c, dtype, 17, level, True, 4)
@@ -358,6 +360,7 @@ def test_fused_batch_at_other_block_sizes(runtime, block):
     note(f"batch graph with {block}-thread workgroups, 6 meshes, 9 iterations")
     batch = M.mesh_batch(nmesh=6, seed=4, cells_lo=300, cells_hi=3000, ordering="random")
     op2.set_loop_config(None, 0, 0, block)
+    op2.set_stage_config(-1)  # stage tiles as res_calc's
     for loop in ("res_calc", "bres_calc"):
         op2.set_loop_config(loop, op2.STRATEGY_OWNER_GATHER, 256, 0)
     for level in (op2.FUSION_STAGES, op2.FUSION_LOOPS):
@@ -379,8 +382,7 @@ def test_stage_kernels_with_two_cells_per_thread(runtime, dtype, renumber):
     """Owner-gather tiles of 512 cells: the stage kernel's threads own two
     cells each; fused equals unfused (q, adt and res bit for bit)."""
     note(f"stage kernels on 512-cell tiles, 60x30 random, {dtype}, renumber={renumber}")
-    for loop in ("res_calc", "bres_calc"):
-        op2.set_loop_config(loop, op2.STRATEGY_OWNER_GATHER, 512, 0)
+    op2.set_stage_config(512, 256)
     mesh = M.o_mesh(60, 30, "random", seed=14)
     c = M.flow_constants().as_array()
     h1, q1, a1, r1 = graph_run(mesh, c, dtype, 11, op2.FUSION_STAGES, renumber, 4)
@@ -389,6 +391,51 @@ def test_stage_kernels_with_two_cells_per_thread(runtime, dtype, renumber):
     h0, q0, a0, r0 = graph_run(mesh, c, dtype, 11, op2.FUSION_OFF, renumber, 4)
     assert same_rms(h1, h0, op2.FUSION_STAGES)
     assert np.array_equal(q1, q0) and np.array_equal(a1, a0) and np.array_equal(r1, r0)
+
+
+@pytest.mark.parametrize("dtype", ["double", "float"])
+@pytest.mark.parametrize("tiles", [(0, 0), (192, 256), (128, 128), (512, 512)])
+def test_stage_tiles_have_their_own_configuration(runtime, dtype, tiles):
+    """The stage kernels run on tiles of their own configuration (default
+    256 cells on 256 threads) while res_calc's eager launches keep 128-cell
+    tiles: the stage plan is built at graph_begin, the stage kernels run
+    (their device times are recorded) and fused equals unfused bit for bit."""
+    part, block = tiles
+    note(f"stage tiles {part}/{block}, res_calc 128-cell tiles, 60x30 random renumbered, "
+         f"{dtype}")
+    mesh = M.o_mesh(60, 30, "random", seed=6)
+    c = M.flow_constants().as_array()
+    op2.set_stage_config(part, block)
+    op2.set_graph_kernel_timing(1)
+    try:
+        h1, q1, a1, r1 = graph_run(mesh, c, dtype, 9, op2.FUSION_STAGES, True, 4)
+    finally:
+        op2.set_graph_kernel_timing(0)
+    op2.op_exit()
+    op2.op_init([], 0)
+    h0, q0, a0, r0 = graph_run(mesh, c, dtype, 9, op2.FUSION_OFF, True, 4)
+    assert same_rms(h1, h0, op2.FUSION_STAGES)
+    assert np.array_equal(q1, q0) and np.array_equal(a1, a0) and np.array_equal(r1, r0)
+    op2.op_exit()
+    op2.op_init([], 0)
+    op2.set_graph_fusion(op2.FUSION_STAGES)
+    af = AF.Airfoil(mesh, c, dtype, renumber=True)
+    af.iterate()
+    n_before = op2.plan_summary("res_calc", af.edges, af.res_calc_args(),
+                                op2.STRATEGY_OWNER_GATHER, 128)["cached_plans"]
+    op2.set_graph_kernel_timing(1)
+    g = af.capture_iteration(2)
+    op2.set_graph_kernel_timing(0)
+    op2.graph_launch(g, 1)
+    times = op2.graph_kernel_times(g)
+    assert len(times) == 4 and all(0 < ms < 1000 for _, ms in times)
+    # the stage plan exists (summary builds nothing more) unless it is res_calc's
+    s = op2.plan_summary("res_calc", af.edges, af.res_calc_args(), op2.STRATEGY_OWNER_GATHER,
+                         part or 256)
+    assert s["verify_errors"] == 0
+    assert s["cached_plans"] == n_before + (part != 128)
+    op2.reduction_log()
+    op2.set_deferred_reductions(False)
 
 
 @pytest.mark.parametrize("dtype", ["double", "float"])
