@@ -127,13 +127,13 @@ def measured_traffic(ni, nj, cfg, keys):
     if (ni, nj) != (2400, 1200) or not cfg or not os.path.exists(path):
         return None
     # keyed rows follow the `config:` line of the runs they come from
-    rows, cfg = {}, None
+    rows, section = {}, None
     for line in open(path):
         if line.startswith("config:"):
-            cfg = tuple(int(v) for v in line.split()[1:4])
+            section = tuple(int(v) for v in line.split()[1:4])
         elif line.startswith("| key:"):
             f = [c.strip() for c in line.split("|")]
-            rows[(cfg, f[1][4:].strip().strip("`"))] = float(f[-4]) * 1e6
+            rows[(section, f[1][4:].strip().strip("`"))] = float(f[-4]) * 1e6
     got = (cfg.get("strategy"), cfg.get("part_size"), cfg.get("block_size"))
     if not keys or any((got, k) not in rows for k in keys):
         return None
