@@ -447,6 +447,8 @@ typedef struct {
   long long verify_errors;
   int max_incs;
   int cached_plans; /* execution plans the runtime holds after this call */
+  int short_ids;    /* OWNER_GATHER: tiles whose positions have 16-bit node and cell
+                       records (each map's targets in two windows of 32768 ids) */
 } op2_plan_summary;
 int op2_plan_summary_get(char const *name, op_set set, int nargs, op_arg *args, int strategy,
                          int part_size, op2_plan_summary *out);

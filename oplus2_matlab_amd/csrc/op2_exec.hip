@@ -398,9 +398,73 @@ __device__ __forceinline__ void ell_sum(const T *fs, int np, ushort4 e4, T acc[4
 typedef int op2_v4i __attribute__((ext_vector_type(4)));
 typedef unsigned short op2_v4us __attribute__((ext_vector_type(4)));
 
+// A target of a 16-bit position record: offset h & 0x7fff in the first of
+// the tile's two windows of its map (base w0), or in the second (base w0 + d)
+// when bit 15 is set.  w0 and d are uniform (scalar registers).
+__device__ __forceinline__ int window_id(unsigned h, int w0, int d) {
+  return w0 + (int)(h & 0x7fffu) + (((int)(h << 16) >> 31) & d);
+}
+
+// The node and cell ids of a position: SHORT, its 8-byte record of 16-bit
+// window offsets (w: the tile's {node base, node step, cell base, cell
+// step}), else its 16-byte record of global ids.
+template <bool SHORT>
+__device__ __forceinline__ op2_v4i position_ids(const int4 *__restrict__ ids,
+                                                const uint16_t *__restrict__ ids16, int4 w,
+                                                int pos) {
+  if (SHORT) {
+    const op2_v4us h = __builtin_nontemporal_load((const op2_v4us *)(ids16 + 4 * (size_t)pos));
+    return op2_v4i{window_id(h.x, w.x, w.y), window_id(h.y, w.x, w.y),
+                   window_id(h.z, w.z, w.w), window_id(h.w, w.z, w.w)};
+  }
+  return __builtin_nontemporal_load((const op2_v4i *)(ids + pos));
+}
+
+// One edge of a gather tile: gathers, flux into LDS (component-major) and
+// the edge's (x1 - x2).  Each gather instruction takes the lower-numbered
+// node (cell) of every position, the next one the higher-numbered:
+// positions follow their smaller cell, so neighbouring lanes then read
+// neighbouring records (fewer L1 lines per instruction than the map's slot
+// order, whose orientation is arbitrary).  The pairs are swapped back in
+// registers, so the flux sees the map's order.
+template <bool SHORT, typename T>
+__device__ __forceinline__ void tile_edge(const int4 *__restrict__ ids,
+                                          const uint16_t *__restrict__ ids16, int4 w,
+                                          const T *__restrict__ x, const T *__restrict__ q,
+                                          const T *__restrict__ adt, const AirfoilConsts<T> &k,
+                                          T *fs, int np, int p0, int p, T side[2]) {
+  const op2_v4i g = position_ids<SHORT>(ids, ids16, w, p0 + p);
+  const int na = min(g.x, g.y), nb = max(g.x, g.y);
+  const int ca = min(g.z, g.w), cb = max(g.z, g.w);
+  T xa[2], xb[2], qa[4], qb[4], f[4];
+  Vec<T>::load2(x + 2 * (size_t)na, xa);
+  Vec<T>::load2(x + 2 * (size_t)nb, xb);
+  Vec<T>::load4(q + 4 * (size_t)ca, qa);
+  Vec<T>::load4(q + 4 * (size_t)cb, qb);
+  const T aa = adt[ca], ab = adt[cb];
+  const bool sn = g.x != na, sc = g.z != ca;
+  T x1[2], x2[2], q1[4], q2[4];
+#pragma unroll
+  for (int d = 0; d < 2; d++) {
+    x1[d] = sn ? xb[d] : xa[d];
+    x2[d] = sn ? xa[d] : xb[d];
+  }
+#pragma unroll
+  for (int d = 0; d < 4; d++) {
+    q1[d] = sc ? qb[d] : qa[d];
+    q2[d] = sc ? qa[d] : qb[d];
+  }
+  edge_flux(x1, x2, q1, q2, sc ? ab : aa, sc ? aa : ab, k, f);
+#pragma unroll
+  for (int d = 0; d < 4; d++) fs[d * np + p] = f[d];
+  side[0] = x1[0] - x2[0];
+  side[1] = x1[1] - x2[1];
+}
+
 template <int NT, typename T, int OWN, bool ELL, bool OWNID>
 __global__ __launch_bounds__(NT) void k_res_calc_owner_direct(
     const int *__restrict__ blk_off, int nblk, const int4 *__restrict__ ids,
+    const uint16_t *__restrict__ ids16, const int4 *__restrict__ id_win,
     const int *__restrict__ owned_off, const int *__restrict__ owned,
     const int *__restrict__ csr_off, const uint16_t *__restrict__ csr,
     const ushort4 *__restrict__ ell, const T *__restrict__ x, const T *__restrict__ q,
@@ -440,38 +504,27 @@ __global__ __launch_bounds__(NT) void k_res_calc_owner_direct(
       Vec<T>::load4(res + 4 * (size_t)gown[j], rpre[j]);
     }
   }
-  for (int p = tid; p < np; p += NT) {
-    const op2_v4i g = __builtin_nontemporal_load((const op2_v4i *)(ids + p0 + p));
-    // Each gather instruction takes the lower-numbered node (cell) of every
-    // position, the next one the higher-numbered: positions follow their
-    // smaller cell, so neighbouring lanes then read neighbouring records
-    // (fewer L1 lines per instruction than the map's slot order, whose
-    // orientation is arbitrary).  The pairs are swapped back in registers,
-    // so the flux sees the map's order.
-    const int na = min(g.x, g.y), nb = max(g.x, g.y);
-    const int ca = min(g.z, g.w), cb = max(g.z, g.w);
-    T xa[2], xb[2], qa[4], qb[4], f[4];
-    Vec<T>::load2(x + 2 * (size_t)na, xa);
-    Vec<T>::load2(x + 2 * (size_t)nb, xb);
-    Vec<T>::load4(q + 4 * (size_t)ca, qa);
-    Vec<T>::load4(q + 4 * (size_t)cb, qb);
-    const T aa = adt[ca], ab = adt[cb];
-    const bool sn = g.x != na, sc = g.z != ca;
-    T x1[2], x2[2], q1[4], q2[4];
-#pragma unroll
-    for (int d = 0; d < 2; d++) {
-      x1[d] = sn ? xb[d] : xa[d];
-      x2[d] = sn ? xa[d] : xb[d];
+  // FP32: tiles with 16-bit records read those (a uniform branch, so each
+  // form's gathers schedule as straight-line code; in FP64 the decode's
+  // registers would cost a wave per SIMD)
+  bool short_done = false;
+  if constexpr (std::is_same<T, float>::value) {
+    if (ids16) {
+      const int4 w = id_win[b];
+      if (w.x >= 0) {
+        for (int p = tid; p < np; p += NT) {
+          T unused[2];
+          tile_edge<true>(ids, ids16, w, x, q, adt, k, fs, np, p0, p, unused);
+        }
+        short_done = true;
+      }
     }
-#pragma unroll
-    for (int d = 0; d < 4; d++) {
-      q1[d] = sc ? qb[d] : qa[d];
-      q2[d] = sc ? qa[d] : qb[d];
-    }
-    edge_flux(x1, x2, q1, q2, sc ? ab : aa, sc ? aa : ab, k, f);
-#pragma unroll
-    for (int d = 0; d < 4; d++) fs[d * np + p] = f[d];
   }
+  if (!short_done)
+    for (int p = tid; p < np; p += NT) {
+      T unused[2];
+      tile_edge<false>(ids, ids16, int4{}, x, q, adt, k, fs, np, p0, p, unused);
+    }
   __syncthreads();
 #pragma unroll
   for (int j = 0; j < OWN; j++) {
@@ -925,38 +978,11 @@ struct StageArgs {
   int nseg;
 };
 
-// One edge of a stage tile: gathers, flux into LDS (component-major), and
-// the edge's (x1 - x2) for the side form of adt_calc.
+// One edge of a stage tile (its 32-bit position ids).
 template <typename T>
 __device__ __forceinline__ void stage_edge(const StageArgs<T> &s, const AirfoilConsts<T> &k,
                                            T *fs, int np, int p0, int p, T side[2]) {
-  const op2_v4i g = __builtin_nontemporal_load((const op2_v4i *)(s.ids + p0 + p));
-  // lower-numbered node and cell first (see k_res_calc_owner_direct)
-  const int na = min(g.x, g.y), nb = max(g.x, g.y);
-  const int ca = min(g.z, g.w), cb = max(g.z, g.w);
-  T xa[2], xb[2], qa[4], qb[4], f[4];
-  Vec<T>::load2(s.x + 2 * (size_t)na, xa);
-  Vec<T>::load2(s.x + 2 * (size_t)nb, xb);
-  Vec<T>::load4(s.q_in + 4 * (size_t)ca, qa);
-  Vec<T>::load4(s.q_in + 4 * (size_t)cb, qb);
-  const T aa = s.adt_in[ca], ab = s.adt_in[cb];
-  const bool sn = g.x != na, sc = g.z != ca;
-  T x1[2], x2[2], q1[4], q2[4];
-#pragma unroll
-  for (int d = 0; d < 2; d++) {
-    x1[d] = sn ? xb[d] : xa[d];
-    x2[d] = sn ? xa[d] : xb[d];
-  }
-#pragma unroll
-  for (int d = 0; d < 4; d++) {
-    q1[d] = sc ? qb[d] : qa[d];
-    q2[d] = sc ? qa[d] : qb[d];
-  }
-  edge_flux(x1, x2, q1, q2, sc ? ab : aa, sc ? aa : ab, k, f);
-#pragma unroll
-  for (int d = 0; d < 4; d++) fs[d * np + p] = f[d];
-  side[0] = x1[0] - x2[0];
-  side[1] = x1[1] - x2[1];
+  tile_edge<false>(s.ids, nullptr, int4{}, s.x, s.q_in, s.adt_in, k, fs, np, p0, p, side);
 }
 
 // With one owned cell per thread the register budget is that of 6 waves
@@ -1675,7 +1701,8 @@ void run_res_calc(const char *name, op_set set, int nargs, op_arg *a) {
     }
     hipLaunchKernelGGL(kern, dim3(p->nblocks), dim3(NT), std::max(lds, (size_t)64),
                        rt().stream, p->d_blk_off, p->nblocks, (const int4 *)p->d_ids,
-                       p->d_owned_off, p->d_owned, p->d_csr_off, p->d_csr,
+                       p->d_ids16, (const int4 *)p->d_id_win, p->d_owned_off, p->d_owned,
+                       p->d_csr_off, p->d_csr,
                        (const ushort4 *)p->d_ell, (const T *)a[0].dat->data_d,
                        (const T *)a[2].dat->data_d, (const T *)a[4].dat->data_d,
                        (T *)a[6].dat->data_d, airfoil_consts<T>(), take_fold(p->nblocks),

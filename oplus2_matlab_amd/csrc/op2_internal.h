@@ -55,6 +55,10 @@ struct PlanGroup {
   int *d_off = nullptr, *d_list = nullptr;
 };
 
+// 16-bit position records of gather tiles: two windows of kIdSpan ids per
+// map per tile; a record is window << 15 | offset in the window.
+constexpr int kIdSpan = 1 << 15;
+
 // Execution plan of one indirect loop on one set (this runtime's design:
 // groups by map rather than by dat, packed 16-bit local indices per
 // (map, idx) slot, elements sorted by thread colour inside each block).
@@ -115,6 +119,17 @@ struct Plan {
   uint16_t *d_loc = nullptr;
   std::vector<int> ids;            // [positions * slot_stride] global targets (gather form)
   int *d_ids = nullptr;
+  // the same targets as 16-bit records (four slots on two maps: slots 0, 1
+  // through the first, 2, 3 through the second), read by the FP32 eager
+  // res_calc; per tile {first window, second - first} of each map, the
+  // target being first + offset (+ second - first for window 1); a tile
+  // whose first base is negative is read through ids.  Empty when no tile
+  // fits.
+  std::vector<uint16_t> ids16;  // [positions * 4]
+  std::vector<int> id_win;      // [nblocks][4]
+  int short_tiles = 0;
+  uint16_t *d_ids16 = nullptr;
+  int *d_id_win = nullptr;
   int *d_owned_off = nullptr, *d_owned = nullptr, *d_csr_off = nullptr;
   uint16_t *d_csr = nullptr, *d_ell = nullptr;
 

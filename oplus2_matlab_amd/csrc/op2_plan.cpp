@@ -63,6 +63,10 @@ void Plan::upload() {
   }
   d_blkmap = (int *)up(blkmap.data(), sizeof(int) * blkmap.size());
   if (!ids.empty()) d_ids = (int *)up(ids.data(), sizeof(int) * ids.size());
+  if (!ids16.empty()) {
+    d_ids16 = (uint16_t *)up(ids16.data(), sizeof(uint16_t) * ids16.size());
+    d_id_win = (int *)up(id_win.data(), sizeof(int) * id_win.size());
+  }
   if (!owned_off.empty()) {
     d_owned_off = (int *)up(owned_off.data(), sizeof(int) * owned_off.size());
     d_owned = (int *)up(owned.data(), sizeof(int) * std::max<size_t>(owned.size(), 1));
@@ -83,9 +87,11 @@ void Plan::release(std::vector<void *> *keep) {
   for (void *p : {(void *)d_blk_off, (void *)d_perm, (void *)d_col_off, (void *)d_col_base,
                   (void *)d_blk_ncol, (void *)d_blkmap, (void *)d_loc, (void *)d_owned_off,
                   (void *)d_owned, (void *)d_csr_off, (void *)d_csr, (void *)d_ell,
-                  (void *)d_ids})
+                  (void *)d_ids, (void *)d_ids16, (void *)d_id_win})
     drop(p);
   d_ids = nullptr;
+  d_ids16 = nullptr;
+  d_id_win = nullptr;
   d_owned_off = d_owned = d_csr_off = nullptr;
   d_csr = d_ell = nullptr;
   for (auto &g : groups) {
@@ -473,6 +479,53 @@ static Plan *build(const PlanRequest &req) {
       for (int s = 0; s < nslots; s++)
         P.ids[pos * P.slot_stride + s] =
             target(P.groups[P.slots[s].first].map, P.slots[s].second, elem((int)pos));
+    // four slots on two maps (res_calc's nodes and cells): 16-bit records.
+    // Each map's targets in a tile are covered greedily, in ascending order,
+    // by at most two windows of kIdSpan ids; a tile that needs more keeps
+    // its 32-bit ids.
+    if (nslots == 4 && P.slots[0].first == P.slots[1].first &&
+        P.slots[2].first == P.slots[3].first && P.slots[0].first != P.slots[2].first) {
+      P.ids16.assign(npos * 4, 0);
+      P.id_win.assign(4 * (size_t)P.nblocks, 0);
+      std::vector<int> t;
+      for (int b = 0; b < P.nblocks; b++) {
+        int *w = &P.id_win[4 * (size_t)b];
+        bool fits = true;
+        for (int m = 0; m < 2 && fits; m++) {
+          t.clear();
+          for (int pos = P.blk_off[b]; pos < P.blk_off[b + 1]; pos++)
+            for (int s = 2 * m; s < 2 * m + 2; s++)
+              t.push_back(P.ids[(size_t)pos * P.slot_stride + s]);
+          if (t.empty()) continue;
+          const auto [lo, hi] = std::minmax_element(t.begin(), t.end());
+          // the second window starts at the first target past the first one
+          int w1 = *hi;
+          for (int v : t)
+            if (v - *lo >= kIdSpan) w1 = std::min(w1, v);
+          fits = *hi - w1 < kIdSpan;
+          if (!fits) break;
+          w[2 * m] = *lo;
+          w[2 * m + 1] = w1 - *lo;
+          for (int pos = P.blk_off[b]; pos < P.blk_off[b + 1]; pos++)
+            for (int s = 2 * m; s < 2 * m + 2; s++) {
+              const int v = P.ids[(size_t)pos * P.slot_stride + s];
+              const bool second = v - *lo >= kIdSpan;
+              P.ids16[(size_t)pos * 4 + s] =
+                  (uint16_t)(second ? 0x8000 | (v - w1) : v - *lo);
+            }
+        }
+        if (fits) {
+          P.short_tiles++;
+        } else {
+          w[0] = -1;
+          w[1] = w[2] = w[3] = 0;
+        }
+      }
+      if (P.short_tiles == 0) {
+        P.ids16.clear();
+        P.id_win.clear();
+      }
+    }
   }
   // kernels read perm only for direct dats, and only if elements moved (the
   // generic kernels take every element through it)
@@ -583,6 +636,26 @@ long long verify_plan(const Plan &P) {
       for (int s = 0; s < nslots; s++)
         err += P.ids[pos * P.slot_stride + s] !=
                target(P.groups[P.slots[s].first].map, P.slots[s].second, P.perm[pos]);
+    // 16-bit records: window base + offset gives the same target
+    if (!P.ids16.empty()) {
+      if (P.ids16.size() != 4 * P.ids.size() / P.slot_stride ||
+          P.id_win.size() != 4 * (size_t)P.nblocks)
+        return err + 1;
+      int short_tiles = 0;
+      for (int b = 0; b < P.nblocks; b++) {
+        const int *w = &P.id_win[4 * (size_t)b];
+        if (w[0] < 0) continue;
+        short_tiles++;
+        for (int pos = P.blk_off[b]; pos < P.blk_off[b + 1]; pos++)
+          for (int s = 0; s < 4; s++) {
+            const unsigned h = P.ids16[(size_t)pos * 4 + s];
+            const int m = s / 2;
+            err += w[2 * m] + (int)(h & 0x7fffu) + ((h >> 15) ? w[2 * m + 1] : 0) !=
+                   P.ids[(size_t)pos * P.slot_stride + s];
+          }
+      }
+      err += short_tiles != P.short_tiles;
+    }
     return err;
   }
   long long err = 0;
@@ -1138,6 +1211,7 @@ int op2_plan_summary_get(char const *name, op_set set, int nargs, op_arg *args, 
     for (auto &g : p->groups) staged += g.max_count;
     out->max_staged = staged;
     out->cached_plans = (int)rt().plans.size();
+    out->short_ids = p->ids16.empty() ? 0 : p->short_tiles;
     return 0;
   } catch (const op2::Error &) {
     return -1;

@@ -67,7 +67,7 @@ class PlanSummary(C.Structure):
     _fields_ = [("nblocks", C.c_int), ("ncolors", C.c_int), ("max_block", C.c_int),
                 ("max_staged", C.c_int), ("strategy", C.c_int), ("part_size", C.c_int),
                 ("mean_thread_colours", C.c_double), ("verify_errors", C.c_longlong),
-                ("max_incs", C.c_int), ("cached_plans", C.c_int)]
+                ("max_incs", C.c_int), ("cached_plans", C.c_int), ("short_ids", C.c_int)]
 
 
 class OpPlan(C.Structure):

@@ -174,6 +174,36 @@ def test_single_loops_fp32_form(runtime, strategy):
     assert not op2.op_fetch_data(b.p_res).any()
 
 
+def test_fp32_res_calc_short_records_keep_the_bits(runtime):
+    """The FP32 eager res_calc on gather tiles reads 16-bit position records
+    where a tile's targets fit two windows and 32-bit ids elsewhere (a
+    renumbered 180k-cell mesh has both).  Each cell sums its increments in
+    its edge order whatever the tiling, so plans of 128, 256 and 512 cells,
+    whose short and full tiles differ, give the same bits."""
+    note("fp32 res_calc, short and full position records, 600x300 random renumbered")
+    mesh = M.o_mesh(600, 300, "random", seed=1)
+    c = M.flow_constants().as_array()
+    rng = np.random.Generator(np.random.PCG64(5))
+    q0 = (np.tile(c[6:10], (mesh.ncell, 1)) *
+          (1 + 0.02 * rng.standard_normal((mesh.ncell, 4)))).astype(np.float32)
+    res0 = (1e-3 * rng.standard_normal((mesh.ncell, 4))).astype(np.float32)
+    outs = []
+    for part in (128, 256, 512):
+        configure(op2.STRATEGY_OWNER_GATHER, part)
+        b = AF.Airfoil(mesh, c, "float", q0=q0, res0=res0, renumber=True)
+        b.adt_calc()
+        b.res_calc()
+        s = op2.plan_summary("res_calc", b.edges, b.res_calc_args(),
+                             op2.STRATEGY_OWNER_GATHER, part)
+        note(f"part {part}: {s['short_ids']} of {s['nblocks']} tiles with 16-bit records")
+        assert s["verify_errors"] == 0 and 0 < s["short_ids"] < s["nblocks"]
+        outs.append(op2.op_fetch_data(b.p_res).reshape(-1, 4))
+        op2.op_exit()
+        op2.op_init([], 0)
+    assert all(np.array_equal(o, outs[0]) for o in outs[1:])
+    assert np.any(outs[0] != res0)
+
+
 # ---------------------------------------------------------------------------
 # full runs against the reference fixtures
 

@@ -168,6 +168,29 @@ def test_owner_tiles_are_compact(host_runtime):
     assert s["max_block"] < 2.5 * 256
 
 
+def test_short_position_records_where_tiles_fit_two_windows(host_runtime):
+    """Gather tiles store 16-bit (window, offset) records when each map's
+    targets in the tile fit two windows of 32768 ids, and keep 32-bit ids
+    otherwise: a random numbering of 320k cells scatters every tile's
+    neighbours over the whole set (only the few-cell tiles left over by
+    breadth-first growth fit), the renumbered mesh keeps them close.  The
+    verifier decodes every short record against the map.  (A layout of this
+    runtime: the reference's plans keep 32-bit maps, common/op_lib_core.c:
+    627-1089.)"""
+    mesh = M.o_mesh(800, 400, "random", seed=5)
+    d = declare(mesh)
+    far = op2.plan_summary("res_calc", d["edges"], d["res_args"], op2.STRATEGY_OWNER_GATHER, 256)
+    assert far["verify_errors"] == 0 and far["short_ids"] < 0.25 * far["nblocks"]
+    op2.renumber(d["pecell"])
+    near = op2.plan_summary("res_calc", d["edges"], d["res_args"],
+                            op2.STRATEGY_OWNER_GATHER, 256)
+    assert near["verify_errors"] == 0
+    assert 0.9 * near["nblocks"] <= near["short_ids"] <= near["nblocks"]
+    # other strategies keep no position ids
+    col = op2.plan_summary("res_calc", d["edges"], d["res_args"], op2.STRATEGY_COLOURED, 256)
+    assert col["short_ids"] == 0
+
+
 # ---------------------------------------------------------------------------
 # renumbering (host side: permutations, internal maps, compact owner tiles)
 
