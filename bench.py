@@ -101,7 +101,7 @@ def parse():
                     help="target CPU time of the cpu_baseline sample (0 = skip)")
     ap.add_argument("--stage-tiles", default="0/0",
                     help="PART/BLOCK of the stage kernels' tiles (0/0 = the runtime's default, "
-                         "256/256; -1/0 = res_calc's configuration)")
+                         "256/256; --stage-tiles=-1/0 = res_calc's configuration)")
     ap.add_argument("--no-fusion", dest="fusion", action="store_false",
                     help="capture the graphs without loop fusion")
     ap.add_argument("--graph-iters", type=int, default=50,

@@ -40,3 +40,11 @@ def test_stage_bytes_of_the_profiled_kinds():
     sq = op2.KIND_BRES | op2.KIND_ADT | op2.KIND_ADT_SIDES | op2.KIND_QOLD_IS_Q
     assert bench.stage_bytes(mesh, si, 8) == pytest.approx(466.6368e6, rel=1e-6)
     assert bench.stage_bytes(mesh, sq, 8) == pytest.approx(374.4768e6, rel=1e-6)
+
+
+def test_stage_tiles_option(monkeypatch):
+    monkeypatch.setattr(sys, "argv", ["bench.py", "--stage-tiles=-1/0", "--steps", "3"])
+    a = bench.parse()
+    assert a.stage_tiles == "-1/0" and a.steps == 3
+    monkeypatch.setattr(sys, "argv", ["bench.py"])
+    assert bench.parse().stage_tiles == "0/0"
