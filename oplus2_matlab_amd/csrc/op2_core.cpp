@@ -739,7 +739,11 @@ int op2_sync(void) {
   API_END(-1)
 }
 
-void *op2_dat_device_ptr(op_dat dat) { return dat ? (void *)dat->data_d : nullptr; }
+void *op2_dat_device_ptr(op_dat dat) {
+  if (!dat) return nullptr;
+  expose_dat(dat);  // other code may now write it: never taken as known zero
+  return (void *)dat->data_d;
+}
 
 int op2_set_deferred_reductions(int on) {
   API_BEGIN
@@ -794,6 +798,7 @@ int op2_graph_begin(void) {
            "before graph capture", o.first.c_str());
   timing_drain();
   reduct_scratch(256 + sizeof(double) * 2048 * 4);
+  forget_zero_dats();  // a replay may follow any eager work
   prepare_capture();
   return_marks(r.capture_marks);
   r.capture_stages = 0;
@@ -868,6 +873,7 @@ int op2_graph_launch(int id, int count) {
   Runtime &r = rt();
   if (id < 0 || id >= (int)r.graphs.size() || !r.graphs[id]) fail("op2: no graph %d", id);
   for (int i = 0; i < count; i++) OP2_HIP_CHECK(hipGraphLaunch(r.graphs[id], r.stream));
+  forget_zero_dats();  // the replays wrote dats
   return 0;
   API_END(-1)
 }

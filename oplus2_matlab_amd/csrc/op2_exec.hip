@@ -1538,19 +1538,27 @@ void run_adt_calc(op_set set, op_arg *a) {
                      airfoil_consts<T>());
 }
 
-// Dats known to hold zeros at this point of a graph capture with fusion on
-// (an update launched earlier in the capture zeroed its res, and nothing
-// has written it since), so that the next res_calc need not read them.
-// Empty at the start of every capture: a graph may be replayed after any
-// eager work.
+// Dats known to hold zeros (an update launched earlier zeroed its res, and
+// no loop has written it since), so that the next owner-compute res_calc
+// need not read them: in eager calls, and in a graph capture with fusion on.
+// Empty at the start of every capture (a graph may be replayed after any
+// eager work) and after anything that writes dats outside op_par_loop.
 std::set<op_dat> &zero_dats() {
+  static std::set<op_dat> z;
+  return z;
+}
+
+// dats whose device pointer the program holds (op2_dat_device_ptr): other
+// code may write them at any time
+std::set<op_dat> &exposed_dats() {
   static std::set<op_dat> z;
   return z;
 }
 
 bool known_zero(op_dat d) {
   const Runtime &r = rt();
-  return r.capturing && r.graph_fusion && zero_dats().count(d);
+  if (r.capturing && !r.graph_fusion) return false;
+  return zero_dats().count(d) && !exposed_dats().count(d);
 }
 
 // A launched loop wrote the dats of its non-READ arguments.
@@ -2976,7 +2984,15 @@ void fusion_flush() {
   flush_fold();
   zero_dats().clear();
 }
+void forget_zero_dats() { zero_dats().clear(); }
+
+void expose_dat(op_dat d) {
+  exposed_dats().insert(d);
+  zero_dats().erase(d);
+}
+
 void fusion_drop() {
+  exposed_dats().clear();
   stage().active = false;
   for (auto &kv : live_buf()) kv.second = nullptr;
   held().active = false;
@@ -3110,6 +3126,7 @@ int run_loop(const char *name, op_set set, int nargs, op_arg *args) {
       return rb == 8 ? run_update<BS, double>(set, args) : run_update<BS, float>(set, args);
     });
     note_writes(nargs, args);
+    if (!rt().capturing) zero_dats().insert(args[2].dat);  // update leaves res zero
     timing_end(t, start);
     if (rt().deferred) return 0;
     gbl_inc(args[4], fetch_results(d, 1)[0]);
@@ -3125,6 +3142,7 @@ int run_loop(const char *name, op_set set, int nargs, op_arg *args) {
     });
     check_launch(name);
     note_writes(nargs, args);
+    if (!rt().capturing) zero_dats().insert(args[2].dat);
     timing_end(t, start);
     return 0;
   }
@@ -3167,6 +3185,7 @@ int autotune(const char *name, op_set set, int nargs, op_arg *args, int reps, in
              int *best_part, double *best_us) {
   ensure_device();
   refuse_with_live_graphs("op2_autotune_loop");
+  forget_zero_dats();  // the trials write and restore the loop's dats
   const Loop l = rt().loops.count(name) ? Loop::kUnknown : identify(name, set, nargs, args);
   if (l != Loop::kResCalc && l != Loop::kBresCalc && l != Loop::kJacRes)
     fail("op2: loop \"%s\" cannot be autotuned (only indirect loops without global results)",
@@ -3358,8 +3377,10 @@ int par_loop_noalloc(const char *name, op_set set, int nargs, op_arg *args) {
     else
       dispatch<float>(l, name, set, nargs, args);
     check_launch(name);
+    note_writes(nargs, args);
     return 0;
   }
+  note_writes(nargs, args);
   const double *d = with_block(block_of(name), [&](auto B) {
     constexpr int BS = decltype(B)::value;
     if (l == Loop::kUpdate)

@@ -274,6 +274,11 @@ void mark_end(int mark);
 // loop fusion during graph capture: launch, or forget, a held loop
 void fusion_flush();
 void fusion_drop();
+// dats known to hold zeros: forgotten when work the runtime does not see
+// may have written them (a graph replay, a restart, a renumbering); a dat
+// whose device pointer the program has taken is never taken as zero
+void forget_zero_dats();
+void expose_dat(op_dat d);
 void h2d(void *dst, const void *src, size_t bytes);
 void d2h(void *dst, const void *src, size_t bytes);
 void ensure_map_device(op_map m);

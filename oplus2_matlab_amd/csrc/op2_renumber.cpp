@@ -356,6 +356,7 @@ void renumber(op_map base) {
     permute_to_internal(d, user.data(), internal.data());
     h2d(d->data_d, internal.data(), bytes);
   }
+  forget_zero_dats();
   if (OP_diags > 1)
     printf(" renumbering: bisection %.2f s, other sets %.2f s, plans %.2f s, maps %.2f s, "
            "dats %.2f s\n", t1 - t0, t2 - t1, t3 - t2, t4 - t3, now() - t4);

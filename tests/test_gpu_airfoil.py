@@ -204,6 +204,52 @@ def test_fp32_res_calc_short_records_keep_the_bits(runtime):
     assert np.any(outs[0] != res0)
 
 
+def test_eager_res_calc_takes_updated_res_as_zero(runtime):
+    """An eager update leaves res zero, so the next owner-gather res_calc
+    starts its sums from +0 without reading res: the same bits as a run in
+    which res is read (its device pointer taken, which stops the runtime from
+    assuming anything about it).  With the pointer taken, values written
+    through it between update and res_calc are added to."""
+    import ctypes as C
+    note("eager res_calc after update, 48x24 random, owner-gather, fp64")
+    mesh = M.o_mesh(48, 24, "random", seed=9)
+    c = M.flow_constants().as_array()
+
+    def run(expose, fill=None):
+        configure(op2.STRATEGY_OWNER_GATHER, 128)
+        b = AF.Airfoil(mesh, c, "double")
+        if expose:
+            ptr = b.p_res.device_ptr()
+        b.save_soln()
+        for _ in range(2):
+            b.adt_calc()
+            b.res_calc()
+            b.bres_calc()
+            b.update()
+        if fill is not None:
+            # the HIP runtime the op2 library loaded
+            path = next((ln.split()[-1] for ln in open("/proc/self/maps")
+                         if "libamdhip64" in ln), "libamdhip64.so")
+            hip = C.CDLL(path)
+            src = np.ascontiguousarray(fill)
+            op2.sync()
+            assert hip.hipMemcpy(C.c_void_p(ptr), src.ctypes.data_as(C.c_void_p),
+                                 C.c_size_t(src.nbytes), 1) == 0
+        b.adt_calc()
+        b.res_calc()
+        res = op2.op_fetch_data(b.p_res).reshape(-1, 4).copy()
+        op2.op_exit()
+        op2.op_init([], 0)
+        return res
+
+    skipped = run(False)
+    read = run(True)
+    assert np.array_equal(skipped, read) and np.any(skipped != 0)
+    ones = np.ones((mesh.ncell, 4))
+    added = run(True, ones)
+    assert np.max(np.abs((added - 1.0) - skipped)) <= 1e-15
+
+
 # ---------------------------------------------------------------------------
 # full runs against the reference fixtures
 
