@@ -22,7 +22,9 @@ runtime places around every 7th stage kernel inside the timed graphs
 (op2_set_graph_kernel_timing), read for the last replay of each graph.
 res_calc_roofline: the same for res_calc alone (76 B/edge FP64), from HIP
 events the runtime records on its stream around every loop call of an eager
-timed region.  traffic = measured HBM bytes per launch from the committed
+timed region; each of those calls follows an update, so the runtime takes
+res as zero and does not read it, and res_read_skipped gives the fraction
+on the bytes the call needs (without that read).  traffic = measured HBM bytes per launch from the committed
 rocprofv3 counter summary, when this run's configuration is the profiled
 one.
 
@@ -452,6 +454,12 @@ def main():
                 "kernel": "res_calc (eager calls)", "bytes_per_launch": rc["bytes_per_call"],
                 "bytes_per_edge": rc["bytes_per_call"] / mesh.nedge,
                 "us_per_launch": rc["us_per_call"]}
+    # every timed res_calc follows an update, which left res zero: the
+    # runtime does not read it (the algorithmic bytes count that read)
+    needed = rc["bytes_per_call"] - 4 * 8 * mesh.ncell
+    res_roof["res_read_skipped"] = {
+        "bytes_per_launch": needed,
+        "frac": needed / (rc["us_per_call"] * 1e-6) / 1e9 / HBM_PEAK_GBS}
     stage = stage_roofline(mesh, r64["stages"], 8)
     stage_cfg = dict(res_cfg or {}) if st_part < 0 else \
         {"strategy": STRATEGIES["owner-gather"], "part_size": st_part or 256,

@@ -429,7 +429,10 @@ int op2_set_stage_config(int part_size, int block_size);
 int op2_set_graph_kernel_timing(int every);
 int op2_graph_kernel_times(int id, int max, int *kinds, float *ms);
 
-/* Device pointer of a dat (for zero-copy interop). */
+/* Device pointer of a dat (for zero-copy interop).  Other code may then
+ * write the dat between loops: from this call on the runtime assumes
+ * nothing about its contents (it otherwise knows, for example, that an
+ * update left res zero, and the next res_calc does not read it). */
 void *op2_dat_device_ptr(op_dat dat);
 
 /* Plan introspection for the plan of the given strategy (COLOURED, ATOMIC,

@@ -22,7 +22,7 @@ def test_measured_traffic_reads_the_rows_of_the_tiles_configuration():
     assert si == pytest.approx(512.53e6) and sq == pytest.approx(420.08e6)
     assert bench.measured_traffic(2400, 1200, stage, ["stage/7si", "stage/5sq"]) == \
         pytest.approx((si + sq) / 2)
-    assert bench.measured_traffic(2400, 1200, res, ["res_calc"]) == pytest.approx(478.65e6)
+    assert bench.measured_traffic(2400, 1200, res, ["res_calc"]) == pytest.approx(386.60e6)
     # unprofiled configurations, meshes and keys give None
     assert bench.measured_traffic(2400, 1200, {"strategy": 5, "part_size": 128,
                                                "block_size": 128}, ["res_calc"]) is None
