@@ -118,6 +118,24 @@ __global__ __launch_bounds__(NT) void k_save_adt(const int4 *__restrict__ pcell,
   adt[c] = adt_of(xs[0], xs[1], xs[2], xs[3], qc, k);
 }
 
+// Sum of p[i] for i = first, first + stride, ... below end, added in that
+// order from +0.0; eight loads are issued before their adds, so that their
+// latencies overlap (the same bits as the plain loop).
+__device__ __forceinline__ double strided_sum(const double *__restrict__ p, int first, int end,
+                                              int stride) {
+  double acc = 0.0;
+  int i = first;
+  for (; i + 7 * stride < end; i += 8 * stride) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) v[u] = p[i + u * stride];
+#pragma unroll
+    for (int u = 0; u < 8; u++) acc += v[u];
+  }
+  for (; i < end; i += stride) acc += p[i];
+  return acc;
+}
+
 // A fold of reduction partials carried by the next kernel of a captured
 // graph instead of a launch of its own.  Single form: workgroup 0 of that
 // kernel sums partials[0 .. n) exactly as k_fold<1> does, into *out and the
@@ -143,14 +161,14 @@ __device__ __forceinline__ void carried_fold(const CarriedFold &f, double *lds) 
   double acc = 0.0;
   if (f.nseg > 0) {
     const int m = blockIdx.x;
-    for (int i = f.pslot[m] + threadIdx.x; i < f.pslot[m + 1]; i += NT) acc += f.partials[i];
+    acc = strided_sum(f.partials, f.pslot[m] + threadIdx.x, f.pslot[m + 1], NT);
     acc = block_reduce<kRedSum, NT>(acc, lds);
     if (threadIdx.x == 0) {
       f.dat[m] += acc;
       if (f.log) f.log[(*f.log_base + m) & (kLogCap - 1)] = acc;
     }
   } else {
-    for (int i = threadIdx.x; i < f.n; i += NT) acc += f.partials[i];
+    acc = strided_sum(f.partials, threadIdx.x, f.n, NT);
     acc = block_reduce<kRedSum, NT>(acc, lds);
     if (threadIdx.x == 0) {
       *f.out = acc;
@@ -1168,8 +1186,7 @@ __global__ __launch_bounds__(kFixedBlock) void k_fold_seg(const double *__restri
   constexpr int NT = kFixedBlock;
   __shared__ double lds[NT / kWave];
   const int m = blockIdx.x;
-  double acc = 0.0;
-  for (int i = pslot[m] + threadIdx.x; i < pslot[m + 1]; i += NT) acc += partials[i];
+  double acc = strided_sum(partials, pslot[m] + threadIdx.x, pslot[m + 1], NT);
   acc = block_reduce<kRedSum, NT>(acc, lds);
   if (threadIdx.x == 0) {
     dat[m] += acc;
@@ -1192,10 +1209,13 @@ __global__ __launch_bounds__(kFixedBlock) void k_fold(const double *__restrict__
 #pragma unroll
   for (int s = 0; s < NSLOT; s++) {
     const bool mx = (maxmask >> s) & 1u;
-    double acc = mx ? red_identity<kRedMax>() : 0.0;
-    for (int i = threadIdx.x; i < nparts; i += NT) {
-      const double v = partials[s * nparts + i];
-      acc = mx ? red_apply<kRedMax>(acc, v) : acc + v;
+    double acc;
+    if (mx) {
+      acc = red_identity<kRedMax>();
+      for (int i = threadIdx.x; i < nparts; i += NT)
+        acc = red_apply<kRedMax>(acc, partials[s * nparts + i]);
+    } else {
+      acc = strided_sum(partials + (size_t)s * nparts, threadIdx.x, nparts, NT);
     }
     acc = mx ? block_reduce<kRedMax, NT>(acc, lds) : block_reduce<kRedSum, NT>(acc, lds);
     if (threadIdx.x == 0) {
@@ -2765,9 +2785,17 @@ void sync_shadows() {
   for (auto &kv : live_buf()) {
     if (!kv.second) continue;
     op_dat d = kv.first;
-    const long long words = (long long)d->size * d->set->size / 4;
-    hipLaunchKernelGGL((k_copy_elems<unsigned>), dim3(grid_for(words, 1 << 16)), dim3(kFixedBlock), 0,
-                       rt().stream, (const unsigned *)kv.second, (unsigned *)d->data_d, words);
+    const long long bytes = (long long)d->size * d->set->size;
+    if (bytes % 16 == 0) {  // 16-byte elements (device buffers are 256-byte aligned)
+      const long long n = bytes / 16;
+      hipLaunchKernelGGL((k_copy_elems<uint4>), dim3(grid_for(n, 1 << 16)), dim3(kFixedBlock), 0,
+                         rt().stream, (const uint4 *)kv.second, (uint4 *)d->data_d, n);
+    } else {
+      const long long words = bytes / 4;
+      hipLaunchKernelGGL((k_copy_elems<unsigned>), dim3(grid_for(words, 1 << 16)),
+                         dim3(kFixedBlock), 0, rt().stream, (const unsigned *)kv.second,
+                         (unsigned *)d->data_d, words);
+    }
     check_launch("graph stage");
     kv.second = nullptr;
   }
